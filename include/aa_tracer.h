@@ -1,0 +1,158 @@
+/*
+ * aa_tracer.h — C ABI of the MI355X (gfx950) reflection tracer.
+ *
+ * This is the drop-in boundary for the reference's GPU tracer backend.  The
+ * reference binds its backend as a Python module with two calls:
+ *   cuda_available(device_id) -> bool             acoustic_agent/steam_rt_cuda.py:33-41
+ *   trace_energy_field_cuda(**kw) -> dict         acoustic_agent/steam_rt_cuda.py:44-226
+ * and builds the scene arrays and BVH it consumes in
+ *   _scene_kernel_arrays(scene)                   acoustic_agent/steam_rt.py:834-924
+ *   build_bvh(bounds_min, bounds_max, leaf_size)  acoustic_agent/bvh.py:8-77
+ * The entry points below replace those calls one for one (see each comment).
+ * All entry points return 0 on success and a negative AA_ERR_* code on
+ * failure; aa_last_error() then holds a message (thread-local).
+ * Pointers named d_* are device pointers (e.g. PyTorch-ROCm tensor data);
+ * the library never frees caller memory.  Every call is reentrant; scene
+ * handles may be shared between host threads.
+ */
+#ifndef AA_TRACER_H
+#define AA_TRACER_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AA_NUM_BANDS 6
+#define AA_MAX_SOURCES 64
+
+enum {
+    AA_OK = 0,
+    AA_ERR_INVALID = -1,  /* bad argument: maps to the reference's ValueError */
+    AA_ERR_DEVICE = -2,   /* HIP runtime failure / no device: RuntimeError */
+    AA_ERR_MEMORY = -3,   /* allocation failed or over budget */
+    AA_ERR_LIMIT = -4,    /* scene or request exceeds a compiled limit */
+    AA_ERR_KERNEL = -5    /* a kernel reported an internal overflow */
+};
+
+/* RNG streams. */
+enum {
+    AA_RNG_ARRAYS = 0,  /* caller-supplied directions [N,3], U [B,N], J [B,N]
+                           (the reference's NumPy streams, steam_rt.py:3958-4044) */
+    AA_RNG_COUNTER = 1  /* Philox4x32-10 keyed by the seed, generated on device */
+};
+
+/* Counters accumulated by one trace (int64 slots of aa_trace_outputs.d_counters). */
+enum {
+    AA_CNT_BOUNCE_QUERIES = 0,
+    AA_CNT_SHADOW_QUERIES = 1,
+    AA_CNT_NODES_BOUNCE = 2,
+    AA_CNT_NODES_SHADOW = 3,
+    AA_CNT_TRIS_BOUNCE = 4,
+    AA_CNT_TRIS_SHADOW = 5,
+    AA_CNT_CONTRIBUTIONS = 6,
+    AA_CNT_ESCAPED = 7,
+    AA_CNT_COUNT = 8
+};
+
+typedef struct aa_scene_s *aa_scene_t;
+
+/* ---- device query (replaces cuda_available, steam_rt_cuda.py:33-41) ---- */
+int aa_device_count(int32_t *count);
+int aa_device_info(int32_t device, char *name, int32_t name_len, char *arch, int32_t arch_len,
+                   uint64_t *hbm_bytes, int32_t *compute_units);
+const char *aa_last_error(void);
+
+/* ---- canonical BVH (replaces build_bvh, bvh.py:8-77) ----
+ * Host-only.  Same outputs as the reference: preorder nodes with escape
+ * links; node arrays sized at least 2n-1, primitives n; stats = {node_count,
+ * leaf_count, max_depth}. */
+int aa_build_bvh(const double *bounds_min, const double *bounds_max, int64_t n, int64_t leaf_size,
+                 double *node_min, double *node_max, int64_t *start, int64_t *count, int64_t *escape,
+                 int64_t *primitives, int64_t *stats);
+
+/* ---- scene (replaces _scene_kernel_arrays + the device input cache,
+ *      steam_rt.py:834-924 and steam_rt_cuda.py:229-261) ----
+ * Triangles carry the reference surface id they tessellate; reflection is
+ * the per-surface energy reflection 1 - alpha [S,6] and scattering the
+ * per-surface mean scattering [S] (steam_rt.py:861,896).  The library builds
+ * the canonical BVH, collapses it to a 4-wide BVH and uploads it. */
+typedef struct {
+    int32_t num_triangles;
+    int32_t num_surfaces;
+    int32_t num_nodes;       /* 4-wide nodes on the device */
+    int32_t depth;           /* 4-wide tree depth */
+    int32_t canonical_nodes; /* binary nodes (reference layout) */
+    int32_t canonical_depth;
+    int32_t leaf_size;
+    int32_t device;
+    uint64_t device_bytes;   /* device memory the scene holds */
+    uint64_t content_hash;   /* FNV-1a over the inputs */
+} aa_scene_info_t;
+
+int aa_scene_create(int32_t device, const float *tri_vertices /*[T,9]*/, const int32_t *tri_surface /*[T]*/,
+                    const float *tri_normals /*[T,3]*/, int32_t num_triangles, const float *reflection /*[S,6]*/,
+                    const float *scattering /*[S]*/, int32_t num_surfaces, int32_t leaf_size, aa_scene_t *out);
+int aa_scene_info(aa_scene_t scene, aa_scene_info_t *info);
+int aa_scene_release(aa_scene_t scene);
+
+/* ---- one trace (replaces trace_energy_field_cuda, steam_rt_cuda.py:44-226) ---- */
+typedef struct {
+    int32_t num_rays;            /* N */
+    int32_t num_bounces;         /* B */
+    int32_t num_bins;            /* K */
+    int32_t num_diffuse_samples; /* M (bank rows) */
+    int32_t num_sources;         /* Ks: 1, or up to AA_MAX_SOURCES keyframe sources
+                                    sharing one listener and one ray set */
+    int32_t rng_mode;            /* AA_RNG_* */
+    int32_t use_bvh;             /* 0: test every triangle (the reference's linear mode) */
+    int32_t render_ambisonics;   /* accumulate the FOA echogram */
+    uint64_t seed;               /* AA_RNG_COUNTER key */
+    float listener[3];
+    float source_forward[3];     /* weighted-dipole axis (directivity.py:58-66) */
+    float bin_duration;
+    float speed_of_sound;
+    float max_path_length;
+    float listener_radius;
+    float source_radius;
+    float irradiance_min_distance;
+    float specular_exponent;
+    float dipole_weight;
+    float dipole_power;
+    int32_t block_size;          /* 0: tuned default; else 64/128/256/512 */
+    int32_t lds_histogram;       /* -1: tuned default; 0: global atomics; 1: LDS privatised */
+    int32_t lds_nodes;           /* -1: tuned default; else top 4-wide nodes staged in LDS */
+    int32_t log_rays;            /* hit-log extent (0: off) */
+    int32_t log_bounces;
+} aa_trace_params;
+
+typedef struct {
+    float *d_echogram;        /* [Ks,6,K], zeroed by the call */
+    float *d_ambisonic;       /* [Ks,6,4,K] or NULL */
+    int32_t *d_hit_counts;    /* [Ks,S] */
+    int32_t *d_contrib_counts;/* [Ks,S] */
+    float *d_surface_energy;  /* [Ks,S] */
+    int32_t *d_actual_bounces;/* [Ks] */
+    int32_t *d_active_count;  /* [Ks] */
+    int64_t *d_counters;      /* [AA_CNT_COUNT] */
+    int32_t *d_log_triangle;  /* [log_rays, log_bounces] hit triangle (-1 none) or NULL */
+    float *d_log_t;           /* [log_rays, log_bounces] or NULL */
+} aa_trace_outputs;
+
+/* d_sources: [Ks,4] device floats (x, y, z, direct_delay_s).
+ * d_directions / d_diffuse_random / d_diffuse_indices: AA_RNG_ARRAYS only.
+ * d_diffuse_bank: [M,3].  stream: hipStream_t or NULL for the null stream.
+ * The call is asynchronous on `stream`. */
+int aa_trace(aa_scene_t scene, const aa_trace_params *params, const float *d_sources, const float *d_directions,
+             const float *d_diffuse_bank, const float *d_diffuse_random, const int32_t *d_diffuse_indices,
+             const aa_trace_outputs *outputs, void *stream);
+
+/* Synchronise `stream` and report a kernel-side overflow flag, if any. */
+int aa_stream_sync(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,427 @@
+"""Pins of the oracle (CPU restatement of the reference tracer).
+
+Each test checks the oracle against something other than its own formula:
+known-answer vectors, closed forms, hand-checked structures, invariants the
+reference guarantees, or physics (Eyring), so that a dropped term, a wrong
+sign or index, or a swapped operand fails here.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from acoustic_agent_amd import sampling, scenes
+from acoustic_agent_amd.scene import BoxSpec, Material, RoomScene, RoomSpec, tessellate
+from oracle import oracle as O
+
+from helpers import Case, band_rel, rel_l1, run_oracle
+
+# ---------------------------------------------------------------------------
+# Philox4x32-10 known answers (Random123 kat_vectors)
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("ctr,key,expected", [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+])
+def test_philox_known_answers(ctr, key, expected):
+    assert tuple(int(x) for x in O.philox(ctr, key)) == expected
+
+
+def test_philox_matches_rocrand_golden_vectors():
+    data = np.load(__import__("pathlib").Path(__file__).parent / "golden" / "philox_rocrand.npz")
+    for ctr, key, out in zip(data["counters"], data["keys"], data["outputs"]):
+        assert np.array_equal(O.philox(ctr, key), out)
+
+
+def test_counter_stream_draws():
+    # directions: unit vectors on the Fibonacci lattice with |jitter| <= 0.02
+    n = 64
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    for i in range(n):
+        d = O.counter_direction(1729, i, n)
+        assert abs(np.linalg.norm(d) - 1.0) < 1e-12
+        assert d[2] == pytest.approx(1.0 - 2.0 * (i + 0.5) / n, abs=1e-15)
+        theta = math.atan2(d[1], d[0])
+        jitter = (theta - (i + 0.5) * golden + math.pi) % (2 * math.pi) - math.pi
+        assert abs(jitter) <= 0.02 + 1e-12
+    # bounce draws: J is the high word of w*M, U has 24 bits
+    u, j = O.counter_bounce(1729, 5, 3, 128)
+    w = O.philox((3, 5, 0, 0), (1729, 0))
+    assert u == int(w[0]) >> 8 and j == (int(w[1]) * 128) >> 32
+
+
+# ---------------------------------------------------------------------------
+# host samplers of the reference ray set (steam_rt.py:3958-4044)
+# ---------------------------------------------------------------------------
+
+
+def test_diffuse_bank_known_answer():
+    bank = sampling.diffuse_sample_bank(4)
+    # radical inverse phi = 0, 1/2, 1/4, 3/4; u = (i + 1/2) / 4
+    expected = np.asarray([
+        (math.sqrt(0.125), 0.0, math.sqrt(0.875)),
+        (-math.sqrt(0.375), 0.0, math.sqrt(0.625)),
+        (0.0, math.sqrt(0.625), math.sqrt(0.375)),
+        (0.0, -math.sqrt(0.875), math.sqrt(0.125)),
+    ])
+    np.testing.assert_allclose(bank, expected, atol=1e-15)
+    big = sampling.diffuse_sample_bank(128)
+    np.testing.assert_allclose(np.linalg.norm(big, axis=1), 1.0, atol=1e-15)
+    assert np.all(big[:, 2] > 0)
+
+
+def test_sphere_samples_use_numpy_jitter_stream():
+    n, seed = 8, 1729
+    dirs = sampling.sphere_samples(n, seed)
+    jitter = np.random.default_rng(seed).uniform(-0.02, 0.02, n)
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    i = np.arange(n) + 0.5
+    theta = np.arctan2(dirs[:, 1], dirs[:, 0])
+    recovered = (theta - i * golden + math.pi) % (2 * math.pi) - math.pi
+    np.testing.assert_allclose(recovered, jitter, atol=1e-12)
+    np.testing.assert_allclose(dirs[:, 2], 1.0 - 2.0 * i / n, atol=1e-15)
+
+
+def test_sphere_samples_fixture():
+    data = np.load(__import__("pathlib").Path(__file__).parent / "golden" / "ray_sets.npz")
+    np.testing.assert_array_equal(sampling.sphere_samples(8, 1729), data["directions_n8_s1729"])
+    U, J = sampling.diffuse_random_sequence(8, 3, 128, 1729)
+    np.testing.assert_array_equal(U, data["random_n8_b3_s1729"])
+    np.testing.assert_array_equal(J, data["indices_n8_b3_s1729"])
+
+
+def test_diffuse_random_sequence_call_order():
+    # per bounce: rng.random(N) then rng.integers(0, M, N) from default_rng(seed + 1)
+    U, J = sampling.diffuse_random_sequence(5, 2, 128, 41)
+    rng = np.random.default_rng(42)
+    r0 = rng.random(5)
+    j0 = rng.integers(0, 128, size=5, dtype=np.int32)
+    r1 = rng.random(5)
+    np.testing.assert_array_equal(U[0], r0)
+    np.testing.assert_array_equal(J[0], j0)
+    np.testing.assert_array_equal(U[1], r1)
+
+
+# ---------------------------------------------------------------------------
+# primitives: analytic known answers, in both analytic and triangle modes
+# ---------------------------------------------------------------------------
+
+_MAT = Material(absorption=(0.1,) * 6, scattering=0.1)
+
+
+def _room(boxes=(), corners=((0.0, 0.0), (5.0, 0.0), (5.0, 4.0), (0.0, 4.0)), height=3.0):
+    return RoomScene(RoomSpec(corners=corners, height=height, wall_material=_MAT, boxes=tuple(boxes)))
+
+
+def _both(room, precision="float64"):
+    arrays = room.kernel_arrays()
+    return [O.OracleScene(arrays, None, precision), O.OracleScene(arrays, tessellate(arrays), precision)]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("origin,direction,surface,t,normal", [
+    ((1, 1, 1.5), (1, 0, 0), 1, 4.0, (-1, 0, 0)),
+    ((1, 1, 1.5), (-1, 0, 0), 3, 1.0, (1, 0, 0)),
+    ((1, 1, 1.5), (0, 0, -1), 4, 1.5, (0, 0, 1)),
+    ((1, 1, 1.5), (0, 0, 1), 5, 1.5, (0, 0, -1)),
+    ((1, 1, 1.5), (0, 1, 0), 2, 3.0, (0, -1, 0)),
+    ((1, 1, 1.5), (1 / math.sqrt(2), 1 / math.sqrt(2), 0), 2, 3 * math.sqrt(2), (0, -1, 0)),
+])
+def test_shoebox_axis_rays(mode, origin, direction, surface, t, normal):
+    scene = _both(_room())[mode]
+    prim, surf, tt, n = scene.closest_hit(origin, direction)
+    assert surf == surface
+    assert tt == pytest.approx(t, rel=1e-12)
+    np.testing.assert_allclose(n, normal, atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ray_from_outside_misses(mode):
+    # tests/test_steam_audio_parity.py:230-239
+    scene = _both(_room(corners=((0, 0), (4, 0), (4, 4), (0, 4)), height=2.8))[mode]
+    prim, surf, t, _ = scene.closest_hit((10.0, 10.0, 1.4), (1.0, 0.0, 0.0))
+    assert surf == -1 and prim == -1 and t >= 1e29
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_box_entry_and_exit(mode):
+    box = BoxSpec(center=(2.5, 2.0), size=(1.0, 1.0, 1.0), rotation_deg=0.0, z_center=0.5, material=_MAT)
+    scene = _both(_room(boxes=[box]))[mode]
+    # from outside: enter the -x face at x = 2.0
+    _, surf, t, n = scene.closest_hit((1.0, 2.0, 0.5), (1.0, 0.0, 0.0))
+    assert surf == 6 and t == pytest.approx(1.0, rel=1e-12)
+    np.testing.assert_allclose(n, (-1, 0, 0), atol=1e-12)
+    # from inside: the exit face at x = 3.0, normal turned against the ray
+    _, surf, t, n = scene.closest_hit((2.5, 2.0, 0.5), (1.0, 0.0, 0.0))
+    assert surf == 6 and t == pytest.approx(0.5, rel=1e-12)
+    np.testing.assert_allclose(n, (-1, 0, 0), atol=1e-12)
+    # from above onto the top face
+    _, surf, t, n = scene.closest_hit((2.5, 2.0, 2.0), (0.0, 0.0, -1.0))
+    assert surf == 6 and t == pytest.approx(1.0, rel=1e-12)
+    np.testing.assert_allclose(n, (0, 0, 1), atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_rotated_box(mode):
+    box = BoxSpec(center=(2.5, 2.0), size=(1.0, 1.0, 1.0), rotation_deg=45.0, z_center=0.5, material=_MAT)
+    scene = _both(_room(boxes=[box]))[mode]
+    # the 45-degree box has a vertex at x = 2.5 - sqrt(0.5); a ray just above
+    # the vertex axis meets the face whose normal is (-1,-1)/sqrt(2)
+    _, surf, t, n = scene.closest_hit((1.0, 2.05, 0.5), (1.0, 0.0, 0.0))
+    assert surf == 6
+    assert t == pytest.approx(1.5 - math.sqrt(0.5) + 0.05, rel=1e-12)
+    np.testing.assert_allclose(np.abs(n), (math.sqrt(0.5), math.sqrt(0.5), 0), atol=1e-12)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_l_polygon_floor(mode):
+    room = _room(corners=scenes.L_OUTLINE, height=2.8)
+    scene = _both(room)[mode]
+    floor = room.num_surfaces - 2
+    _, surf, t, _ = scene.closest_hit((3.0, 7.0, 1.5), (0.0, 0.0, -1.0))
+    assert surf == floor and t == pytest.approx(1.5)
+    # (10, 7) is in the notch of the L: below it there is no floor
+    _, surf, _, _ = scene.closest_hit((10.0, 7.0, 1.5), (0.0, 0.0, -1.0))
+    assert surf == -1
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_parallel_ray_skips_wall(mode):
+    scene = _both(_room())[mode]
+    # along wall_0 (y = 0 plane) from inside: parallel to it, so it hits wall_1
+    _, surf, t, _ = scene.closest_hit((1.0, 1e-3, 1.5), (1.0, 0.0, 0.0))
+    assert surf == 1 and t == pytest.approx(4.0)
+
+
+# ---------------------------------------------------------------------------
+# BVH: hand-checked trees (bvh.py:8-77), product == oracle, BVH == linear
+# ---------------------------------------------------------------------------
+
+
+def _boxes_at(xs):
+    lo = np.asarray([[x - 0.1, 0.0, 0.0] for x in xs], dtype=float)
+    return lo, lo + np.asarray([0.2, 1.0, 1.0])
+
+
+def _both_builders():
+    from acoustic_agent_amd.bvh import build_bvh
+
+    return [build_bvh, O.build_bvh]
+
+
+@pytest.mark.parametrize("builder", _both_builders())
+def test_bvh_known_trees(builder):
+    b = builder(*_boxes_at([0.0]))
+    assert b["node_count"] == 1 and list(b["count"]) == [1] and list(b["escape"]) == [1]
+    b = builder(*_boxes_at([3.0, 1.0, 2.0, 0.0]))
+    assert b["node_count"] == 1 and list(b["primitives"]) == [0, 1, 2, 3]
+    b = builder(*_boxes_at([4.0, 3.0, 2.0, 1.0, 0.0]))
+    assert list(b["start"]) == [-1, 0, 2] and list(b["count"]) == [0, 2, 3]
+    assert list(b["escape"]) == [3, 2, 3] and list(b["primitives"]) == [3, 4, 0, 1, 2]
+    assert b["max_depth"] == 2 and b["leaf_count"] == 2
+    b = builder(*_boxes_at([float(x) for x in range(9)]))
+    assert list(b["count"]) == [0, 4, 0, 2, 3]
+    assert list(b["escape"]) == [5, 2, 5, 4, 5]
+    assert list(b["primitives"]) == list(range(9))
+    assert b["max_depth"] == 3
+    np.testing.assert_allclose(b["bounds_min"][0], (-0.1, 0.0, 0.0))
+    np.testing.assert_allclose(b["bounds_max"][0], (8.1, 1.0, 1.0))
+
+
+@pytest.mark.parametrize("builder", _both_builders())
+def test_bvh_axis_and_stability(builder):
+    # equal extents on x and y: argmax keeps x; equal centroids keep input order
+    lo = np.asarray([[0, 0, 0], [0, 0, 0], [1, 1, 0], [1, 1, 0], [0.5, 0.5, 0]], dtype=float)
+    b = builder(lo, lo + 0.1)
+    assert list(b["primitives"]) == [0, 1, 2, 3, 4]
+    assert list(b["count"]) == [0, 2, 3]
+
+
+def test_bvh_validation():
+    from acoustic_agent_amd.bvh import build_bvh
+
+    with pytest.raises(ValueError):
+        build_bvh(np.zeros((0, 3)), np.zeros((0, 3)))
+    with pytest.raises(ValueError):
+        build_bvh(np.ones((2, 3)), np.zeros((2, 3)))
+    with pytest.raises(ValueError):
+        build_bvh(np.full((2, 3), np.nan), np.zeros((2, 3)))
+    with pytest.raises(ValueError):
+        build_bvh(np.zeros((2, 2)), np.zeros((2, 2)))
+
+
+def test_product_and_oracle_builders_agree_on_l_plan():
+    from acoustic_agent_amd.bvh import build_bvh
+
+    w = scenes.l_plan_workload(seed=3, target_triangles=5000)
+    lo, hi = w.mesh.bounds(2e-6)
+    a, b = build_bvh(lo, hi), O.build_bvh(lo, hi)
+    for key in ("bounds_min", "bounds_max", "start", "count", "escape", "primitives"):
+        np.testing.assert_array_equal(a[key], b[key])
+    assert (a["node_count"], a["leaf_count"], a["max_depth"]) == (b["node_count"], b["leaf_count"], b["max_depth"])
+
+
+@pytest.mark.parametrize("triangles", [False, True])
+def test_bvh_closest_hit_equals_linear(triangles):
+    w = scenes.l_plan_workload(seed=1, target_triangles=3000)
+    arrays = w.scene.kernel_arrays()
+    mesh = w.mesh if triangles else None
+    bvh = O.OracleScene(arrays, mesh, "float64", use_bvh=True)
+    lin = O.OracleScene(arrays, mesh, "float64", use_bvh=False)
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        o = np.asarray([rng.uniform(0.2, 11.8), rng.uniform(0.2, 4.8), rng.uniform(0.1, 2.7)])
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        a, b = bvh.closest_hit(o, d), lin.closest_hit(o, d)
+        assert a[0] == b[0] and a[1] == b[1] and a[2] == b[2]
+
+
+# ---------------------------------------------------------------------------
+# whole traces
+# ---------------------------------------------------------------------------
+
+
+@pytest.fixture(scope="module")
+def lplan():
+    return scenes.l_plan_workload(seed=0, target_triangles=6000)
+
+
+def test_triangles_reproduce_analytic_reference_primitives(lplan):
+    case = Case(lplan, num_rays=1500, num_bounces=60, num_bins=2000, seed=3, log_bounces=60)
+    a = run_oracle(case, "float64", triangles=False, log_rays=1500)
+    t = run_oracle(case, "float64", triangles=True, log_rays=1500)
+    same = np.all(a["log_surface"] == t["log_surface"], axis=1)
+    assert same.mean() >= 0.998
+    assert np.all(band_rel(t["echogram"], a["echogram"]) <= 1e-3)
+    shoebox = Case(scenes.shoebox_workload(0), num_rays=2000, num_bounces=50, num_bins=1000, seed=1, log_bounces=50)
+    a = run_oracle(shoebox, "float64", triangles=False, log_rays=2000)
+    t = run_oracle(shoebox, "float64", triangles=True, log_rays=2000)
+    assert np.array_equal(a["log_surface"], t["log_surface"])
+    assert np.array_equal(a["hit_counts"], t["hit_counts"])
+    assert rel_l1(t["echogram"], a["echogram"]) <= 1e-12
+
+
+def test_shadow_any_hit_equals_closest_hit(lplan):
+    case = Case(lplan, num_rays=800, num_bounces=40, num_bins=2000, seed=5)
+    dirs, bank, U, J = case.ray_set()
+    scene = O.OracleScene(case.arrays, lplan.mesh, "float64")
+    kw = dict(source=lplan.source, listener=lplan.listener, num_rays=800, num_bounces=40, num_bins=2000,
+              bin_dur=1e-3, diffuse_bank=bank, directions=dirs, diffuse_random=U, diffuse_indices=J)
+    a = O.trace(scene, shadow_closest=False, **kw)
+    b = O.trace(scene, shadow_closest=True, **kw)
+    np.testing.assert_array_equal(a["echogram"], b["echogram"])
+    np.testing.assert_array_equal(a["contrib_counts"], b["contrib_counts"])
+    assert a["counters"]["nodes_shadow"] <= b["counters"]["nodes_shadow"]
+
+
+def test_thread_count_only_reorders_sums(lplan):
+    case = Case(lplan, num_rays=1000, num_bounces=40, num_bins=2000, seed=8)
+    a = run_oracle(case, "float64", threads=1)
+    b = run_oracle(case, "float64", threads=7)
+    np.testing.assert_array_equal(a["hit_counts"], b["hit_counts"])
+    np.testing.assert_allclose(a["echogram"], b["echogram"], rtol=1e-12, atol=1e-18)
+    assert a["active_count"] == b["active_count"] and a["actual_bounces"] == b["actual_bounces"]
+
+
+def test_ray_ranges_partition_the_trace(lplan):
+    case = Case(lplan, num_rays=900, num_bounces=30, num_bins=2000, seed=9)
+    whole = run_oracle(case, "float64")
+    parts = [run_oracle(case, "float64", ray_range=r) for r in ((0, 300), (300, 900))]
+    np.testing.assert_allclose(parts[0]["echogram"] + parts[1]["echogram"], whole["echogram"], rtol=1e-12,
+                               atol=1e-18)
+    np.testing.assert_array_equal(parts[0]["hit_counts"] + parts[1]["hit_counts"], whole["hit_counts"])
+
+
+def test_single_ray_deposit_closed_form():
+    """One ray straight down in a shoebox; the NEE geometry is analytic.
+
+    Listener (2.5, 2, 1.5), ray (0,0,-1): floor hit at t = 1.5, hit point
+    h = (2.5, 2, 0.01) after the 1 cm offset; source straight above at
+    (2.5, 2, 2.8): D = 2.79, facing = 1, half vector = +z (cos = 1).
+    Bin: ((1.5 + 2.79) - 1.3) / c / 1 ms.  The specular continuation goes back
+    up through the listener sphere and ends the ray at bounce 1.
+    """
+    sigma = 0.2
+    alpha = np.asarray([0.05, 0.1, 0.15, 0.2, 0.25, 0.3])
+    floor = Material(absorption=tuple(alpha), scattering=sigma)
+    room = RoomScene(RoomSpec(corners=((0, 0), (5, 0), (5, 4), (0, 4)), height=3.0, wall_material=_MAT,
+                              floor_material=floor))
+    arrays = room.kernel_arrays()
+    src, lis = np.asarray((2.5, 2.0, 2.8)), np.asarray((2.5, 2.0, 1.5))
+    bank = sampling.diffuse_sample_bank(128)
+    for mesh in (None, tessellate(arrays)):
+        scene = O.OracleScene(arrays, mesh, "float64")
+        res = O.trace(scene, source=src, listener=lis, num_rays=1, num_bounces=5, num_bins=1000, bin_dur=1e-3,
+                      diffuse_bank=bank, directions=np.asarray([[0.0, 0.0, -1.0]]),
+                      diffuse_random=np.full((5, 1), 0.99), diffuse_indices=np.zeros((5, 1), dtype=np.int32))
+        D = 2.79
+        bin_index = int(math.floor(((1.5 + D) - 1.3) / 343.0 / 1e-3))
+        assert bin_index == 8
+        brdf = sigma / math.pi + 102.0 / (8.0 * math.pi) * (1.0 - sigma)
+        expected = 4.0 * math.pi * (1.0 / (4.0 * math.pi * D * D)) * brdf * (1.0 - alpha)
+        nz = np.flatnonzero(res["echogram"].sum(axis=0))
+        assert list(nz) == [bin_index]
+        np.testing.assert_allclose(res["echogram"][:, bin_index], expected, rtol=1e-12)
+        assert res["hit_counts"][4] == 1 and res["contrib_counts"][4] == 1
+        assert res["active_count"] == 0 and res["actual_bounces"] == 2
+
+
+def test_fp32_oracle_matches_fp64_at_reference_tolerance():
+    # tests/test_accelerators.py:31-53: 256 rays, 6 bounces, 0.3 s, rtol 2e-4
+    room = scenes.shoebox((6.0, 4.0, 2.8), seed=2)
+    w = scenes.Workload(room, tessellate(room.kernel_arrays()), np.asarray((1.0, 1.0, 1.4)),
+                        np.asarray((4.5, 2.7, 1.4)), 0, None)
+    case = Case(w, num_rays=256, num_bounces=6, num_bins=30, bin_dur=0.01, seed=1729)
+    for triangles in (False, True):
+        a = run_oracle(case, "float32", triangles=triangles)
+        b = run_oracle(case, "float64", triangles=triangles)
+        np.testing.assert_allclose(a["echogram"], b["echogram"], rtol=2e-4, atol=1e-8)
+        np.testing.assert_array_equal(a["hit_counts"], b["hit_counts"])
+
+
+def test_foa_channels_carry_the_initial_direction():
+    # tests/test_steam_audio_parity.py:329-357
+    w = scenes.Workload(scenes.shoebox((6.0, 4.0, 2.8), 0), None, np.asarray((4.0, 2.0, 1.4)),
+                        np.asarray((1.0, 1.0, 1.4)), 0, None)
+    arrays = w.scene.kernel_arrays()
+    dirs = sampling.sphere_samples(1, 0)
+    U, J = sampling.diffuse_random_sequence(1, 12, 128, 0)
+    res = O.trace(O.OracleScene(arrays, None, "float64"), source=w.source, listener=w.listener, num_rays=1,
+                  num_bounces=12, num_bins=100, bin_dur=0.01, diffuse_bank=sampling.diffuse_sample_bank(128),
+                  directions=dirs, diffuse_random=U, diffuse_indices=J, render_ambisonics=True)
+    e = res["echogram"][0]
+    valid = e > 0
+    assert np.count_nonzero(valid) > 1
+    np.testing.assert_allclose(res["ambisonic"][0, 0, valid] / e[valid], 1.0, atol=1e-12)
+    for ch in range(3):
+        np.testing.assert_allclose(res["ambisonic"][0, ch + 1, valid] / e[valid], dirs[0, ch], rtol=1e-12,
+                                   atol=1e-12)
+
+
+def test_diffuse_shoebox_rt60_matches_eyring():
+    # benchmark.py:268-298 (full profile: 131072 rays, 96 bounces, 2 s, 10% of Eyring)
+    from acoustic_agent_amd.models import SimConfig
+    from acoustic_agent_amd.steam_rt import estimate_reverb_times
+
+    alpha = 0.25
+    mat = Material(absorption=(alpha,) * 6, scattering=0.05)
+    size = (6.0, 4.0, 2.8)
+    room = RoomScene(RoomSpec(corners=((0, 0), (6, 0), (6, 4), (0, 4)), height=2.8, wall_material=mat))
+    w = scenes.Workload(room, None, np.asarray((1.3, 1.1, 1.4)), np.asarray((4.7, 2.9, 1.4)), 0, None)
+    case = Case(w, num_rays=131072, num_bounces=96, num_bins=200, bin_dur=0.01, seed=1729)
+    res = run_oracle(case, "float64", triangles=False)
+    field = {"echogram": res["echogram"], "bin_duration_s": 0.01, "num_bins": 200}
+    rt = estimate_reverb_times(field, SimConfig())
+    volume = size[0] * size[1] * size[2]
+    surface = 2.0 * (size[0] * size[1] + size[0] * size[2] + size[1] * size[2])
+    eyring = 0.161 * volume / (-surface * math.log(1.0 - alpha))
+    measured = float(np.median([v for v in rt.values() if v > 0]))
+    assert len([v for v in rt.values() if v > 0]) == 6
+    assert abs(measured - eyring) / eyring <= 0.10, (measured, eyring)
