@@ -98,6 +98,30 @@ class TraceOutputs(ctypes.Structure):
     ]
 
 
+class BatchTrace(ctypes.Structure):
+    _fields_ = [
+        ("scene", ctypes.c_void_p),
+        ("params", TraceParams),
+        ("d_sources", ctypes.c_void_p),
+        ("d_directions", ctypes.c_void_p),
+        ("d_diffuse_bank", ctypes.c_void_p),
+        ("d_diffuse_random", ctypes.c_void_p),
+        ("d_diffuse_indices", ctypes.c_void_p),
+        ("outputs", TraceOutputs),
+    ]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [
+        ("block_size", ctypes.c_int32),
+        ("packet_rays", ctypes.c_int32),
+        ("lds_nodes", ctypes.c_int32),
+        ("lds_histogram", ctypes.c_int32),
+        ("workgroups_per_cu", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+    ]
+
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -115,6 +139,9 @@ _SIGNATURES = {
     "aa_scene_release": (_I32, [_P]),
     "aa_trace": (_I32, [_P, ctypes.POINTER(TraceParams), _P, _P, _P, _P, _P, ctypes.POINTER(TraceOutputs), _P]),
     "aa_stream_sync": (_I32, [_P]),
+    "aa_batch_create": (_I32, [_I32, _I32, ctypes.POINTER(_P)]),
+    "aa_batch_submit": (_I32, [_P, ctypes.POINTER(BatchTrace), _I32, ctypes.POINTER(BatchTuning), _P]),
+    "aa_batch_release": (_I32, [_P]),
 }
 
 _lock = threading.Lock()
@@ -140,6 +167,14 @@ def load() -> ctypes.CDLL:
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
                 "(the tracer has no CPU fallback)"
             )
+        # PyTorch-ROCm ships its own libamdhip64.so.7 / libhsa-runtime64.so.1.
+        # Loading torch first makes this library bind to that same runtime
+        # (same SONAME); loading it first would bring a second HIP runtime into
+        # the process and torch would then see no device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = ctypes.CDLL(os.fspath(LIB_PATH))
         for name, (restype, argtypes) in _SIGNATURES.items():
             fn = getattr(lib, name)

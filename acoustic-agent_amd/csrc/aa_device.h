@@ -1,0 +1,747 @@
+// Device code shared by the single-trace kernel (aa_tracer.hip) and the
+// persistent batch kernel (aa_batch.hip): BVH traversal, triangle test,
+// counter RNG and the per-ray bounce loop of the reference tracer
+// (acoustic_agent/steam_rt.py:4724-4849).
+//
+// Floating-point contract: built with -ffp-contract=off and IEEE division /
+// sqrt, so every trajectory operation rounds like the binary32 CPU
+// restatement; only the BVH box tests (culling only: results do not depend on
+// them) use explicit fused multiply-adds.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/aa_tracer.h"
+#include "aa_internal.h"
+
+namespace aa {
+
+constexpr int kBands = AA_NUM_BANDS;
+constexpr int kStack = 40;          // per-lane traversal stack entries (uint16 node indices in LDS)
+constexpr int kMaxWideNodes = 65535; // node indices must fit the 16-bit stack entries
+constexpr float kEps = 1e-6f;       // steam_rt_cuda.py:19
+constexpr float kHitOffset = 1e-2f; // steam_rt_cuda.py:20
+constexpr float kBvhEps = 1e-5f;    // steam_rt_cuda.py:21
+constexpr float kBig = 1e30f;
+
+// LDS bytes of the per-lane stacks, rounded up to 16 so the float4 node
+// staging that follows stays aligned.
+template <int WG>
+__host__ __device__ constexpr size_t kStackBytes() {
+    return ((size_t)kStack * WG * sizeof(unsigned short) + 15) & ~(size_t)15;
+}
+inline size_t stack_bytes(int wg) { return ((size_t)kStack * wg * sizeof(unsigned short) + 15) & ~(size_t)15; }
+
+// A device-resident triangle scene: 4-wide BVH nodes (8 x float4 each),
+// triangles in leaf order (3 x float4: v0|tri id, e1|surface id, e2|0),
+// per-triangle normals (original order) and per-surface materials.
+struct SceneView {
+    const float4* nodes;
+    const float4* tris;
+    const float4* normals;
+    const float* reflection;
+    const float* scattering;
+    int num_nodes, num_tris, num_surfaces, use_bvh;
+    int lds_nodes;  // nodes [0, lds_nodes) are read from LDS
+};
+
+// Per-trace ray set and deposit parameters.
+struct RayParams {
+    const float* directions;
+    const float* bank;
+    const float* urand;
+    const int* uidx;
+    const float4* sources;  // x, y, z, direct delay
+    int num_rays, num_bounces, num_bins, num_samples, num_sources, rng_mode, amb;
+    unsigned seed_lo, seed_hi;
+    float lx, ly, lz, fx, fy, fz;
+    float bin_dur, c, max_path_len, listener_radius, source_radius, dmin, spec_exp, dip_w, dip_p;
+    float ray_scale;
+};
+
+struct Outputs {
+    float* echogram;
+    float* ambisonic;
+    int* hit_counts;
+    int* contrib_counts;
+    float* surface_energy;
+    int* actual_bounces;
+    int* active_count;
+    unsigned long long* counters;
+    int* log_tri;
+    float* log_t;
+    int log_rays, log_bounces;
+};
+
+// Per-lane tallies flushed once per wave.
+struct LaneStats {
+    unsigned q_b = 0, q_s = 0, nb = 0, ns = 0, tb = 0, ts = 0, contrib = 0, escaped = 0;
+    int max_iter_full = 0;  // bounce iterations of rays that ended with every source alive
+    int active_full = 0;
+};
+
+// LDS views of one workgroup.
+struct Lds {
+    unsigned short* stack;  // kStack x WG
+    const float4* nodes; // staged top nodes
+    float* hist;         // [Ks,6,K] or null
+    int* hits;           // [S] or null (single source only)
+    int* contrib;
+    float* senergy;
+};
+
+__device__ __forceinline__ void philox4x32_10(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0,
+                                              unsigned k1, unsigned& r0, unsigned& r1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const unsigned hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        const unsigned hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        const unsigned n0 = hi1 ^ c1 ^ k0;
+        const unsigned n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+    }
+    r0 = c0;
+    r1 = c1;
+}
+
+__device__ __forceinline__ float safe_inv(float d) {
+    // Axis with |d| <= 1e-12: the slab test must only require the origin to
+    // lie inside the slab (steam_rt_cuda.py:416-419); a huge signed inverse
+    // gives exactly that and never produces a NaN.
+    return fabsf(d) > 1e-12f ? 1.0f / d : copysignf(1e30f, d);
+}
+
+__device__ __forceinline__ bool tri_test(const float4 a, const float4 b, const float4 c, float ox, float oy, float oz,
+                                         float dx, float dy, float dz, float& t_out) {
+    // Moller-Trumbore in the operand order of the binary32 CPU restatement.
+    const float e1x = b.x, e1y = b.y, e1z = b.z;
+    const float e2x = c.x, e2y = c.y, e2z = c.z;
+    const float px = dy * e2z - dz * e2y;
+    const float py = dz * e2x - dx * e2z;
+    const float pz = dx * e2y - dy * e2x;
+    const float det = e1x * px + e1y * py + e1z * pz;
+    if (!(fabsf(det) > 1e-12f)) return false;
+    const float inv = 1.0f / det;
+    const float sx = ox - a.x, sy = oy - a.y, sz = oz - a.z;
+    const float u = (sx * px + sy * py + sz * pz) * inv;
+    if (u < -1e-6f || u > 1.000001f) return false;
+    const float qx = sy * e1z - sz * e1y;
+    const float qy = sz * e1x - sx * e1z;
+    const float qz = sx * e1y - sy * e1x;
+    const float v = (dx * qx + dy * qy + dz * qz) * inv;
+    if (v < -1e-6f || u + v > 1.000001f) return false;
+    const float t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+    if (!(t > kEps)) return false;
+    t_out = t;
+    return true;
+}
+
+struct Ray {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+};
+
+struct Closest {
+    float t;
+    int surf;
+    int tri;
+};
+
+__device__ __forceinline__ bool better(float t, int surf, int tri, const Closest& h) {
+    // (t, surface, triangle) lexicographic: the reference's tie rule
+    // steam_rt.py:4499 extended by the triangle index so that the result does
+    // not depend on traversal order.
+    if (t < h.t) return true;
+    if (t == h.t) {
+        if (h.surf < 0 || surf < h.surf) return true;
+        if (surf == h.surf && tri < h.tri) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ void load_node(const SceneView& sv, const float4* s_nodes, int node, float4* v) {
+    if (node < sv.lds_nodes) {
+#pragma unroll
+        for (int r = 0; r < 7; ++r) v[r] = s_nodes[node * 8 + r];
+    } else {
+        const float4* g = sv.nodes + (size_t)node * 8;
+#pragma unroll
+        for (int r = 0; r < 7; ++r) v[r] = g[r];
+    }
+}
+
+__device__ __forceinline__ void slab4(const float4* v, const Ray& r, float tlim, float* tn, bool* hit) {
+    const float oix = r.ox * r.ix, oiy = r.oy * r.iy, oiz = r.oz * r.iz;
+    const float lxs[4] = {v[0].x, v[0].y, v[0].z, v[0].w};
+    const float hxs[4] = {v[1].x, v[1].y, v[1].z, v[1].w};
+    const float lys[4] = {v[2].x, v[2].y, v[2].z, v[2].w};
+    const float hys[4] = {v[3].x, v[3].y, v[3].z, v[3].w};
+    const float lzs[4] = {v[4].x, v[4].y, v[4].z, v[4].w};
+    const float hzs[4] = {v[5].x, v[5].y, v[5].z, v[5].w};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float tx0 = __builtin_fmaf(lxs[c], r.ix, -oix), tx1 = __builtin_fmaf(hxs[c], r.ix, -oix);
+        const float ty0 = __builtin_fmaf(lys[c], r.iy, -oiy), ty1 = __builtin_fmaf(hys[c], r.iy, -oiy);
+        const float tz0 = __builtin_fmaf(lzs[c], r.iz, -oiz), tz1 = __builtin_fmaf(hzs[c], r.iz, -oiz);
+        const float tmin = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
+        const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tlim));
+        tn[c] = tmin;
+        hit[c] = tmin <= tmax && tmax > kEps;
+    }
+}
+
+__device__ __forceinline__ void test_tri(const SceneView& sv, int k, const Ray& r, Closest& h) {
+    const float4* t4 = sv.tris + (size_t)k * 3;
+    const float4 ta = t4[0], tb = t4[1], tc = t4[2];
+    float t;
+    if (tri_test(ta, tb, tc, r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, t)) {
+        const int tri = __float_as_int(ta.w), surf = __float_as_int(tb.w);
+        if (better(t, surf, tri, h)) {
+            h.t = t;
+            h.surf = surf;
+            h.tri = tri;
+        }
+    }
+}
+
+template <int WG>
+__device__ void closest_hit(const SceneView& sv, const float4* s_nodes, unsigned short* stk, const Ray& r, Closest& h,
+                            unsigned& node_cnt, unsigned& tri_cnt, int* error_flag) {
+    h.t = kBig;
+    h.surf = -1;
+    h.tri = -1;
+    if (!sv.use_bvh) {
+        for (int k = 0; k < sv.num_tris; ++k) {
+            ++tri_cnt;
+            test_tri(sv, k, r, h);
+        }
+        return;
+    }
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        ++node_cnt;
+        float4 v[7];
+        load_node(sv, s_nodes, node, v);
+        float tn[4];
+        bool hit[4];
+        slab4(v, r, h.t + kBvhEps, tn, hit);
+        const int ch[4] = {__float_as_int(v[6].x), __float_as_int(v[6].y), __float_as_int(v[6].z),
+                           __float_as_int(v[6].w)};
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (hit[c] && ch[c] <= -2) {
+                const int code = -(ch[c] + 2);
+                const int start = code >> 3, count = (code & 7) + 1;
+                for (int k = 0; k < count; ++k) {
+                    ++tri_cnt;
+                    test_tri(sv, start + k, r, h);
+                }
+            }
+        }
+        // internal children, nearest first
+        float key[4];
+        int idx[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const bool ok = hit[c] && ch[c] >= 0 && tn[c] <= h.t + kBvhEps;
+            key[c] = ok ? tn[c] : INFINITY;
+            idx[c] = ch[c];
+        }
+#define AA_CSWAP(i, j)         \
+    if (key[j] < key[i]) {     \
+        const float tk = key[i]; \
+        key[i] = key[j];       \
+        key[j] = tk;           \
+        const int ti = idx[i]; \
+        idx[i] = idx[j];       \
+        idx[j] = ti;           \
+    }
+        AA_CSWAP(0, 1)
+        AA_CSWAP(2, 3)
+        AA_CSWAP(0, 2)
+        AA_CSWAP(1, 3)
+        AA_CSWAP(1, 2)
+#undef AA_CSWAP
+#pragma unroll
+        for (int c = 3; c >= 1; --c) {
+            if (key[c] < INFINITY) {
+                if (sp < kStack) {
+                    stk[sp * WG] = (unsigned short)idx[c];
+                    ++sp;
+                } else {
+                    atomicOr(error_flag, 1);
+                }
+            }
+        }
+        if (key[0] < INFINITY) {
+            node = idx[0];
+        } else if (sp > 0) {
+            --sp;
+            node = (int)stk[sp * WG];
+        } else {
+            break;
+        }
+    }
+}
+
+template <int WG>
+__device__ bool any_hit(const SceneView& sv, const float4* s_nodes, unsigned short* stk, const Ray& r, float max_distance,
+                        unsigned& node_cnt, unsigned& tri_cnt, int* error_flag) {
+    const float lo = kEps, hi = max_distance - kEps;
+    if (!sv.use_bvh) {
+        for (int k = 0; k < sv.num_tris; ++k) {
+            const float4* t4 = sv.tris + (size_t)k * 3;
+            ++tri_cnt;
+            float t;
+            if (tri_test(t4[0], t4[1], t4[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, t) && t > lo && t < hi) return true;
+        }
+        return false;
+    }
+    int sp = 0;
+    int node = 0;
+    const float tlim = max_distance + kBvhEps;
+    while (true) {
+        ++node_cnt;
+        float4 v[7];
+        load_node(sv, s_nodes, node, v);
+        float tn[4];
+        bool hit[4];
+        slab4(v, r, tlim, tn, hit);
+        const int ch[4] = {__float_as_int(v[6].x), __float_as_int(v[6].y), __float_as_int(v[6].z),
+                           __float_as_int(v[6].w)};
+        int next = -1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            if (!hit[c]) continue;
+            if (ch[c] <= -2) {
+                const int code = -(ch[c] + 2);
+                const int start = code >> 3, count = (code & 7) + 1;
+                for (int k = 0; k < count; ++k) {
+                    const float4* t4 = sv.tris + (size_t)(start + k) * 3;
+                    ++tri_cnt;
+                    float t;
+                    if (tri_test(t4[0], t4[1], t4[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, t) && t > lo && t < hi)
+                        return true;
+                }
+            } else if (ch[c] >= 0) {
+                if (next < 0) {
+                    next = ch[c];
+                } else if (sp < kStack) {
+                    stk[sp * WG] = (unsigned short)ch[c];
+                    ++sp;
+                } else {
+                    atomicOr(error_flag, 1);
+                }
+            }
+        }
+        if (next >= 0) {
+            node = next;
+        } else if (sp > 0) {
+            --sp;
+            node = (int)stk[sp * WG];
+        } else {
+            return false;
+        }
+    }
+}
+
+__device__ __forceinline__ bool sphere_before(float ox, float oy, float oz, float dx, float dy, float dz, float cx,
+                                              float cy, float cz, float radius, float max_distance) {
+    const float px = ox - cx, py = oy - cy, pz = oz - cz;
+    const float b = px * dx + py * dy + pz * dz;
+    const float cc = px * px + py * py + pz * pz - radius * radius;
+    const float disc = b * b - cc;
+    if (disc < 0.0f) return false;
+    const float root = -b - sqrtf(disc > 0.0f ? disc : 0.0f);
+    return root >= 0.0f && root < max_distance;
+}
+
+__device__ __forceinline__ void normalize3(float& x, float& y, float& z) {
+    const float n = sqrtf(x * x + y * y + z * z);
+    if (n <= 1e-12f) {
+        x = y = z = 0.0f;
+        return;
+    }
+    x = x / n;
+    y = y / n;
+    z = z / n;
+}
+
+__device__ __forceinline__ void initial_direction(const RayParams& p, int ray, float& dx, float& dy, float& dz) {
+    if (p.rng_mode == AA_RNG_ARRAYS) {
+        dx = p.directions[(size_t)ray * 3];
+        dy = p.directions[(size_t)ray * 3 + 1];
+        dz = p.directions[(size_t)ray * 3 + 2];
+        return;
+    }
+    // Fibonacci lattice with a U(-0.02, 0.02) jitter (steam_rt.py:3958-3968), in binary64.
+    unsigned w0, w1;
+    philox4x32_10(0xFFFFFFFFu, (unsigned)ray, 0u, 0u, p.seed_lo, p.seed_hi, w0, w1);
+    const double u = (double)(w0 >> 8) * (1.0 / 16777216.0);
+    const double jitter = -0.02 + 0.04 * u;
+    const double n = (double)(p.num_rays > 0 ? p.num_rays : 1);
+    const double i = (double)ray + 0.5;
+    const double z = 1.0 - 2.0 * i / n;
+    double rr = 1.0 - z * z;
+    rr = rr < 0.0 ? 0.0 : (rr > 1.0 ? 1.0 : rr);
+    const double rad = sqrt(rr);
+    const double theta = i * (M_PI * (3.0 - 2.23606797749978969640917366873127623544061835961152572)) + jitter;
+    dx = (float)(cos(theta) * rad);
+    dy = (float)(sin(theta) * rad);
+    dz = (float)z;
+}
+
+// Trace one listener ray through its bounces and deposit its next-event
+// contributions.  LDS_HIST: histogram privatised in lds.hist, else global
+// atomics into out.echogram.  lds.hits != null: single-source per-surface
+// tallies privatised in LDS.
+template <int WG, bool LDS_HIST>
+__device__ void trace_one_ray(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds, int ray,
+                              LaneStats& st, int* error_flag) {
+    unsigned short* stk = lds.stack + threadIdx.x;
+    Ray r;
+    r.ox = p.lx;
+    r.oy = p.ly;
+    r.oz = p.lz;
+    initial_direction(p, ray, r.dx, r.dy, r.dz);
+    const float d0x = r.dx, d0y = r.dy, d0z = r.dz;
+    float A[kBands];
+#pragma unroll
+    for (int b = 0; b < kBands; ++b) A[b] = 1.0f;
+    float accum = 0.0f;
+    const int K = p.num_bins;
+    const int Ks = p.num_sources;
+    const unsigned long long full_mask = Ks >= 64 ? ~0ull : ((1ull << Ks) - 1ull);
+    unsigned long long mask = full_mask;
+    bool alive = true;
+    int iter = 0;
+    const bool logging = out.log_tri != nullptr && ray < out.log_rays;
+    for (int bounce = 0; bounce < p.num_bounces; ++bounce) {
+        iter = bounce + 1;
+        r.ix = safe_inv(r.dx);
+        r.iy = safe_inv(r.dy);
+        r.iz = safe_inv(r.dz);
+        Closest h;
+        ++st.q_b;
+        closest_hit<WG>(sv, lds.nodes, stk, r, h, st.nb, st.tb, error_flag);
+        if (logging && bounce < out.log_bounces) {
+            out.log_tri[(size_t)ray * out.log_bounces + bounce] = h.tri;
+            out.log_t[(size_t)ray * out.log_bounces + bounce] = h.t;
+        }
+        if (h.surf < 0) ++st.escaped;
+        if (h.surf < 0 || h.t <= p.listener_radius || accum > p.max_path_len) {
+            alive = false;
+            break;
+        }
+        if (bounce > 0) {
+            if (sphere_before(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, p.lx, p.ly, p.lz, p.listener_radius, h.t)) {
+                alive = false;
+                break;
+            }
+            unsigned long long m = mask;
+            while (m) {
+                const int k = __builtin_ctzll(m);
+                m &= m - 1ull;
+                const float4 S = p.sources[k];
+                const float sdist =
+                    sqrtf((p.lx - S.x) * (p.lx - S.x) + (p.ly - S.y) * (p.ly - S.y) + (p.lz - S.z) * (p.lz - S.z));
+                if (sdist > p.source_radius &&
+                    sphere_before(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, S.x, S.y, S.z, p.source_radius, h.t)) {
+                    mask &= ~(1ull << k);
+                    atomicMax(&out.actual_bounces[k], bounce + 1);
+                }
+            }
+            if (mask == 0ull) {
+                alive = false;
+                break;
+            }
+        }
+        const float4 nn = sv.normals[h.tri];
+        float nx = nn.x, ny = nn.y, nz = nn.z;
+        if (nx * r.dx + ny * r.dy + nz * r.dz > 0.0f) {
+            nx = -nx;
+            ny = -ny;
+            nz = -nz;
+        }
+        const int surf = h.surf;
+        const float t = h.t;
+        const float hx = r.ox + t * r.dx + kHitOffset * nx;
+        const float hy = r.oy + t * r.dy + kHitOffset * ny;
+        const float hz = r.oz + t * r.dz + kHitOffset * nz;
+        const float scat = sv.scattering[surf];
+        float refl[kBands];
+#pragma unroll
+        for (int b = 0; b < kBands; ++b) refl[b] = sv.reflection[surf * kBands + b];
+        unsigned long long m = mask;
+        while (m) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1ull;
+            if (lds.hits)
+                atomicAdd(&lds.hits[surf], 1);
+            else
+                atomicAdd(&out.hit_counts[(size_t)k * sv.num_surfaces + surf], 1);
+            const float4 S = p.sources[k];
+            const float tsx = S.x - hx, tsy = S.y - hy, tsz = S.z - hz;
+            const float D = sqrtf(tsx * tsx + tsy * tsy + tsz * tsz);
+            if (!(D > p.dmin)) continue;
+            const float sdx = tsx / D, sdy = tsy / D, sdz = tsz / D;
+            const float facing = nx * sdx + ny * sdy + nz * sdz;
+            if (!(facing > 0.0f)) continue;
+            Ray sr;
+            sr.ox = hx;
+            sr.oy = hy;
+            sr.oz = hz;
+            sr.dx = sdx;
+            sr.dy = sdy;
+            sr.dz = sdz;
+            sr.ix = safe_inv(sdx);
+            sr.iy = safe_inv(sdy);
+            sr.iz = safe_inv(sdz);
+            ++st.q_s;
+            if (any_hit<WG>(sv, lds.nodes, stk, sr, D, st.ns, st.ts, error_flag)) continue;
+            const float diffuse = 0.31830988618379067f * scat * facing;
+            float hvx = sdx - r.dx, hvy = sdy - r.dy, hvz = sdz - r.dz;
+            normalize3(hvx, hvy, hvz);
+            float cos_half = hvx * nx + hvy * ny + hvz * nz;
+            cos_half = cos_half < 0.0f ? 0.0f : cos_half;
+            const float specular =
+                (p.spec_exp + 2.0f) * 0.039788735772973836f * (1.0f - scat) * powf(cos_half, p.spec_exp);
+            const float dm = D > p.dmin ? D : p.dmin;
+            const float distance_term = 0.07957747154594767f / (dm * dm);
+            const float source_cosine = -(p.fx * sdx + p.fy * sdy + p.fz * sdz);
+            const float source_gain = powf(fabsf((1.0f - p.dip_w) + p.dip_w * source_cosine), p.dip_p);
+            const float rel_delay = (accum + t + D) / p.c - S.w;
+            const float q = rel_delay / p.bin_dur;
+            if (!(q >= 0.0f && q < (float)K)) continue;
+            const int bin = (int)floorf(q);
+            const float base = p.ray_scale * source_gain * distance_term * (diffuse + specular);
+            float esum = 0.0f;
+#pragma unroll
+            for (int b = 0; b < kBands; ++b) {
+                const float e = base * refl[b] * A[b];
+                const size_t hi = ((size_t)k * kBands + b) * K + bin;
+                if (LDS_HIST)
+                    atomicAdd(&lds.hist[hi], e);
+                else
+                    atomicAdd(&out.echogram[hi], e);
+                if (p.amb) {
+                    float* am = out.ambisonic + (((size_t)k * kBands + b) * 4) * K + bin;
+                    atomicAdd(am, e);
+                    atomicAdd(am + K, e * d0x);
+                    atomicAdd(am + 2 * K, e * d0y);
+                    atomicAdd(am + 3 * K, e * d0z);
+                }
+                esum += e;
+            }
+            ++st.contrib;
+            if (lds.hits) {
+                atomicAdd(&lds.contrib[surf], 1);
+                atomicAdd(&lds.senergy[surf], esum);
+            } else {
+                atomicAdd(&out.contrib_counts[(size_t)k * sv.num_surfaces + surf], 1);
+                atomicAdd(&out.surface_energy[(size_t)k * sv.num_surfaces + surf], esum);
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kBands; ++b) A[b] *= refl[b];
+        accum += t;
+        r.ox = hx;
+        r.oy = hy;
+        r.oz = hz;
+        float u;
+        int j;
+        if (p.rng_mode == AA_RNG_ARRAYS) {
+            u = p.urand[(size_t)bounce * p.num_rays + ray];
+            j = p.uidx[(size_t)bounce * p.num_rays + ray];
+        } else {
+            unsigned w0, w1;
+            philox4x32_10((unsigned)bounce, (unsigned)ray, 0u, 0u, p.seed_lo, p.seed_hi, w0, w1);
+            u = (float)(w0 >> 8) * (1.0f / 16777216.0f);
+            j = (int)(((unsigned long long)w1 * (unsigned long long)(unsigned)p.num_samples) >> 32);
+        }
+        if (u < scat) {
+            const float sx = p.bank[j * 3], sy = p.bank[j * 3 + 1], sz = p.bank[j * 3 + 2];
+            float hx2 = 1.0f, hy2 = 0.0f, hz2 = 0.0f;
+            if (fabsf(nx) >= 0.9f) {
+                hx2 = 0.0f;
+                hy2 = 1.0f;
+            }
+            float tx = hy2 * nz - hz2 * ny;
+            float ty = hz2 * nx - hx2 * nz;
+            float tz = hx2 * ny - hy2 * nx;
+            normalize3(tx, ty, tz);
+            const float bx = ny * tz - nz * ty;
+            const float by = nz * tx - nx * tz;
+            const float bz = nx * ty - ny * tx;
+            float ndx = sx * tx + sy * bx + sz * nx;
+            float ndy = sx * ty + sy * by + sz * ny;
+            float ndz = sx * tz + sy * bz + sz * nz;
+            normalize3(ndx, ndy, ndz);
+            r.dx = ndx;
+            r.dy = ndy;
+            r.dz = ndz;
+        } else {
+            const float dot = r.dx * nx + r.dy * ny + r.dz * nz;
+            float ndx = r.dx - 2.0f * dot * nx;
+            float ndy = r.dy - 2.0f * dot * ny;
+            float ndz = r.dz - 2.0f * dot * nz;
+            normalize3(ndx, ndy, ndz);
+            r.dx = ndx;
+            r.dy = ndy;
+            r.dz = ndz;
+        }
+        if (accum > p.max_path_len) {
+            alive = false;
+            break;
+        }
+    }
+    if (mask == full_mask) {
+        st.max_iter_full = max(st.max_iter_full, iter);
+        if (alive) ++st.active_full;
+    } else {
+        unsigned long long m = mask;
+        while (m) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1ull;
+            atomicMax(&out.actual_bounces[k], iter);
+            if (alive) atomicAdd(&out.active_count[k], 1);
+        }
+    }
+}
+
+// Wave reduction of the lane tallies, one atomic per wave and counter.
+__device__ __forceinline__ void flush_stats(LaneStats& st, const Outputs& out, int num_sources) {
+    unsigned long long vals[8] = {st.q_b, st.q_s, st.nb, st.ns, st.tb, st.ts, st.contrib, st.escaped};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        unsigned long long v = vals[c];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        vals[c] = v;
+    }
+    int mi = st.max_iter_full, ac = st.active_full;
+    for (int off = 32; off > 0; off >>= 1) {
+        mi = max(mi, __shfl_xor(mi, off));
+        ac += __shfl_xor(ac, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (vals[c]) atomicAdd(&out.counters[c], vals[c]);
+        for (int k = 0; k < num_sources; ++k) {
+            if (mi) atomicMax(&out.actual_bounces[k], mi);
+            if (ac) atomicAdd(&out.active_count[k], ac);
+        }
+    }
+    st = LaneStats();
+}
+
+// ---------------------------------------------------------------------------
+// host-side construction of the kernel argument structs
+// ---------------------------------------------------------------------------
+
+inline SceneView make_scene_view(const aa_scene_s* s, bool use_bvh, int lds_nodes) {
+    SceneView sv;
+    sv.nodes = reinterpret_cast<const float4*>(s->d_nodes);
+    sv.tris = reinterpret_cast<const float4*>(s->d_tris);
+    sv.normals = reinterpret_cast<const float4*>(s->d_normals);
+    sv.reflection = s->d_reflection;
+    sv.scattering = s->d_scattering;
+    sv.num_nodes = (int)s->wide.nodes.size();
+    sv.num_tris = s->num_triangles;
+    sv.num_surfaces = s->num_surfaces;
+    sv.use_bvh = use_bvh ? 1 : 0;
+    sv.lds_nodes = use_bvh ? lds_nodes : 0;
+    return sv;
+}
+
+inline RayParams make_ray_params(const aa_trace_params* p, const float* sources, const float* directions,
+                                 const float* bank, const float* urand, const int* uidx) {
+    RayParams r;
+    r.directions = directions;
+    r.bank = bank;
+    r.urand = urand;
+    r.uidx = uidx;
+    r.sources = reinterpret_cast<const float4*>(sources);
+    r.num_rays = p->num_rays;
+    r.num_bounces = p->num_bounces;
+    r.num_bins = p->num_bins;
+    r.num_samples = p->num_diffuse_samples;
+    r.num_sources = p->num_sources;
+    r.rng_mode = p->rng_mode;
+    r.amb = p->render_ambisonics ? 1 : 0;
+    r.seed_lo = (unsigned)(p->seed & 0xffffffffull);
+    r.seed_hi = (unsigned)(p->seed >> 32);
+    r.lx = p->listener[0];
+    r.ly = p->listener[1];
+    r.lz = p->listener[2];
+    r.fx = p->source_forward[0];
+    r.fy = p->source_forward[1];
+    r.fz = p->source_forward[2];
+    r.bin_dur = p->bin_duration;
+    r.c = p->speed_of_sound;
+    r.max_path_len = p->max_path_length;
+    r.listener_radius = p->listener_radius;
+    r.source_radius = p->source_radius;
+    r.dmin = p->irradiance_min_distance;
+    r.spec_exp = p->specular_exponent;
+    r.dip_w = p->dipole_weight;
+    r.dip_p = p->dipole_power;
+    // (float)(4 pi) / N, as the reference FP32 kernel (steam_rt_cuda.py:570)
+    r.ray_scale = 12.566370614359172f / (float)(p->num_rays > 1 ? p->num_rays : 1);
+    return r;
+}
+
+inline Outputs make_outputs(const aa_trace_outputs* o, const aa_trace_params* p) {
+    Outputs out;
+    out.echogram = o->d_echogram;
+    out.ambisonic = o->d_ambisonic;
+    out.hit_counts = o->d_hit_counts;
+    out.contrib_counts = o->d_contrib_counts;
+    out.surface_energy = o->d_surface_energy;
+    out.actual_bounces = o->d_actual_bounces;
+    out.active_count = o->d_active_count;
+    out.counters = reinterpret_cast<unsigned long long*>(o->d_counters);
+    const bool log = p->log_rays > 0 && p->log_bounces > 0;
+    out.log_rays = log ? (p->log_rays < p->num_rays ? p->log_rays : p->num_rays) : 0;
+    out.log_bounces = log ? p->log_bounces : 0;
+    out.log_tri = log ? o->d_log_triangle : nullptr;
+    out.log_t = log ? o->d_log_t : nullptr;
+    return out;
+}
+
+// Byte extents of one trace's outputs (for zeroing).
+struct OutputExtents {
+    void* ptr[8];
+    size_t bytes[8];
+};
+
+inline OutputExtents output_extents(const aa_trace_outputs* o, const aa_trace_params* p, int num_surfaces) {
+    const size_t Ks = (size_t)p->num_sources, K = (size_t)p->num_bins, S = (size_t)num_surfaces;
+    OutputExtents e;
+    e.ptr[0] = o->d_echogram;
+    e.bytes[0] = Ks * AA_NUM_BANDS * K * 4;
+    e.ptr[1] = p->render_ambisonics ? o->d_ambisonic : nullptr;
+    e.bytes[1] = Ks * AA_NUM_BANDS * 4 * K * 4;
+    e.ptr[2] = o->d_hit_counts;
+    e.bytes[2] = Ks * S * 4;
+    e.ptr[3] = o->d_contrib_counts;
+    e.bytes[3] = Ks * S * 4;
+    e.ptr[4] = o->d_surface_energy;
+    e.bytes[4] = Ks * S * 4;
+    e.ptr[5] = o->d_actual_bounces;
+    e.bytes[5] = Ks * 4;
+    e.ptr[6] = o->d_active_count;
+    e.bytes[6] = Ks * 4;
+    e.ptr[7] = o->d_counters;
+    e.bytes[7] = AA_CNT_COUNT * 8;
+    return e;
+}
+
+}  // namespace aa

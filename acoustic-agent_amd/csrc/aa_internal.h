@@ -6,6 +6,10 @@
 #include <string>
 #include <vector>
 
+#include "../../include/aa_tracer.h"
+
+struct aa_scene_s;
+
 namespace aa {
 
 // Canonical binary BVH in the reference layout (acoustic_agent/bvh.py:8-77):
@@ -51,4 +55,38 @@ struct WideBVH {
 // the largest-area internal child first), laid out breadth first.
 void collapse_to_wide(const CanonicalBVH& bin, WideBVH& out);
 
+
+// Record a message for aa_last_error() and return `code`.
+int set_error(int code, const char* fmt, ...);
+
+struct DeviceProps {
+    int cu = 0;
+    size_t lds_per_block = 0;
+    bool valid = false;
+};
+const DeviceProps& device_props(int device);
+
+// Validate one trace request (shared by aa_trace and the batch path).
+int validate_trace(const ::aa_scene_s* scene, const aa_trace_params* p, const float* d_sources,
+                   const float* d_directions, const float* d_diffuse_bank, const float* d_diffuse_random,
+                   const int32_t* d_diffuse_indices, const aa_trace_outputs* o);
+
 }  // namespace aa
+
+// Opaque scene handle of the C ABI.
+struct aa_scene_s {
+    int32_t device = 0;
+    int32_t num_triangles = 0, num_surfaces = 0;
+    int32_t leaf_size = 4;
+    int32_t canonical_nodes = 0, canonical_depth = 0;
+    aa::WideBVH wide;
+    float* d_nodes = nullptr;      // [nodes, 8, 4]
+    float* d_tris = nullptr;       // [T, 3, 4] leaf order
+    float* d_normals = nullptr;    // [T, 4]
+    float* d_reflection = nullptr; // [S, 6]
+    float* d_scattering = nullptr; // [S]
+    int* d_error = nullptr;
+    uint64_t device_bytes = 0;
+    uint64_t content_hash = 0;
+};
+

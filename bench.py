@@ -1,0 +1,222 @@
+"""Benchmark of the MI355X reflection tracer.
+
+Default workload (one GPU): the plain single-scene trace of the furnished
+three-room L plan (12 x 9 x 2.8 m, 60 seeded boxes, ~20k triangles), 200,000
+listener rays, 200 bounces, 2.0 s at 1 ms bins x 6 bands, binary32, counter
+RNG.  A step is one full trace (one RIR's echogram) with the scene and all
+buffers resident in HBM.  The JSON line reports RIRs/s (``value``), primary
+rays/s, segment queries/s, BVH nodes/s, per-step p50/p99 latency, the
+roofline of the trace kernel (algorithmic bytes from the oracle's canonical
+traversal counters, HIP-event kernel time) and the CPU baseline: the binary64
+oracle of the same trace on the host cores, whose echogram also gives the
+band RT60 / EDT error of the GPU result.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+from __graft_entry__ import load_package  # noqa: E402
+
+load_package()
+from acoustic_agent_amd import hip_backend, sampling, scenes  # noqa: E402
+from acoustic_agent_amd import _lib  # noqa: E402
+from acoustic_agent_amd.models import SimConfig  # noqa: E402
+from acoustic_agent_amd.steam_rt import estimate_early_decay_times, estimate_reverb_times  # noqa: E402
+
+C = 343.0
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_NODE = 32   # FP32 AABB (24 B) + packed start/count/escape (8 B)
+BYTES_PER_TRI = 48    # 3 x FP32 vertices + id / padding
+BYTES_PER_CONTRIB = 48  # 6 bands x 8 B histogram read-modify-write
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def band_errors(a: np.ndarray, b: np.ndarray, bin_dur: float) -> dict[str, float]:
+    cfg = SimConfig()
+    out = {}
+    for name, est in (("rt60", estimate_reverb_times), ("edt", estimate_early_decay_times)):
+        fa = est({"echogram": a, "bin_duration_s": bin_dur, "num_bins": a.shape[-1]}, cfg)
+        fb = est({"echogram": b, "bin_duration_s": bin_dur, "num_bins": b.shape[-1]}, cfg)
+        errs = [abs(fa[k] - fb[k]) / fb[k] for k in fb if fb[k] > 0]
+        out[name + "_max_rel_err"] = float(max(errs)) if errs else None
+        out[name + "_oracle_s"] = {k: round(float(v), 5) for k, v in fb.items()}
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rays", type=int, default=200_000)
+    ap.add_argument("--bounces", type=int, default=200)
+    ap.add_argument("--triangles", type=int, default=20_000)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget-s", type=float, default=25.0, help="bound on the CPU baseline's wall time")
+    ap.add_argument("--tuning", type=str, default="", help="JSON dict of launch overrides")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build (traces are independent "
+              "and would shard by trace across GPUs)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    if not hip_backend.hip_available(0):
+        print("no HIP device", file=sys.stderr)
+        return 1
+    tuning = json.loads(args.tuning) if args.tuning else None
+    N, B, bin_dur, duration = int(args.rays), int(args.bounces), 1e-3, 2.0
+    K = int(round(duration / bin_dur))
+    w = scenes.l_plan_workload(seed=args.seed, target_triangles=args.triangles)
+    arrays = w.scene.kernel_arrays()
+    scene = hip_backend.DeviceScene(w.mesh, np.asarray(arrays["reflection"]), np.asarray(arrays["scattering"]),
+                                    device=0)
+    dev = "cuda:0"
+    bank = torch.from_numpy(np.ascontiguousarray(sampling.diffuse_sample_bank(128), dtype=np.float32)).to(dev)
+    delay = float(np.linalg.norm(w.source - w.listener)) / C
+    src = torch.tensor([[*w.source, delay]], dtype=torch.float32, device=dev)
+    buffers = hip_backend.TraceBuffers.allocate(0, 1, K, scene.num_surfaces, False)
+    params = hip_backend.make_params(
+        num_rays=N, num_bounces=B, num_bins=K, num_diffuse_samples=128, num_sources=1, rng_mode=_lib.AA_RNG_COUNTER,
+        use_bvh=True, render_ambisonics=False, seed=args.seed, listener=w.listener, source_forward=(1.0, 0.0, 0.0),
+        bin_dur=bin_dur, speed_of_sound=C, max_path_len=duration * C, listener_radius=0.1, source_radius=0.1,
+        irradiance_min_distance=1.0, specular_exponent=100.0, dipole_weight=0.0, dipole_power=1.0, tuning=tuning)
+    stream = torch.cuda.current_stream(0)
+
+    def step():
+        hip_backend.launch_trace(scene, params, src, bank, buffers, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ev0, ev1 in events:
+        ev0.record(stream)
+        step()
+        ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = [a.elapsed_time(b) for a, b in events]
+    ms_per_step = wall * 1000.0 / args.steps
+    counters = dict(zip(_lib.COUNTER_NAMES, (int(x) for x in buffers.counters.cpu().numpy())))
+    gpu_echogram = buffers.echogram.cpu().numpy()[0].astype(np.float64)
+    avg_kernel_s = statistics.mean(kernel_ms) / 1000.0
+
+    result = {
+        "metric": "RIRs/s + rays/s at <=2% band-RT60 err vs FP64 oracle; p50/p99 served latency",
+        "value": round(1000.0 / ms_per_step, 3),
+        "unit": "RIRs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded scene and counter-RNG ray set)",
+        "config": {
+            "workload": "plain single-scene trace: furnished 3-room L plan 12x9x2.8 m, 60 boxes, "
+                        f"{w.mesh.num_triangles} triangles, {N} rays, {B} bounces, 2.0 s @ 1 ms x 6 bands",
+            "triangles": w.mesh.num_triangles,
+            "surfaces": w.scene.num_surfaces,
+            "rays": N,
+            "bounces": B,
+            "bins": K,
+            "rng": "counter (Philox4x32-10)",
+            "tuning": tuning or "default",
+        },
+        "primary_rays_per_s": round(N / (ms_per_step / 1000.0), 1),
+        "segment_queries_per_s": round((counters["bounce_queries"] + counters["shadow_queries"]) / avg_kernel_s, 1),
+        "bvh4_nodes_per_s": round((counters["nodes_bounce"] + counters["nodes_shadow"]) / avg_kernel_s, 1),
+        "latency_ms": {"p50": round(float(np.percentile(kernel_ms, 50)), 4),
+                       "p99": round(float(np.percentile(kernel_ms, 99)), 4)},
+        "kernel_ms_avg": round(avg_kernel_s * 1000.0, 4),
+        "gpu_counters": counters,
+        "scene": {k: scene.info[k] for k in ("num_nodes", "depth", "canonical_nodes", "device_bytes")},
+    }
+
+    roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(REPO))
+        from oracle import oracle as O
+
+        cores = len(os.sched_getaffinity(0))
+        oscene = O.OracleScene(arrays, w.mesh, "float64")
+        kw = dict(source=w.source, listener=w.listener, num_rays=N, num_bounces=B, num_bins=K, bin_dur=bin_dur, c=C,
+                  max_path_len=duration * C, direct_delay=delay, diffuse_bank=sampling.diffuse_sample_bank(128),
+                  rng="counter", seed=args.seed, threads=cores)
+        probe_rays = min(N, 4000)
+        t = time.perf_counter()
+        O.trace(oscene, ray_range=(0, probe_rays), **kw)
+        probe_s = time.perf_counter() - t
+        est_full = probe_s * N / probe_rays
+        sample = N if est_full <= args.cpu_budget_s else max(probe_rays, int(N * args.cpu_budget_s / est_full))
+        t = time.perf_counter()
+        ref = O.trace(oscene, ray_range=(0, sample), **kw)
+        cpu_s = time.perf_counter() - t
+        oc = ref["counters"]
+        node_b = oc["nodes_bounce"] / max(oc["bounce_queries"], 1)
+        tri_b = oc["prims_bounce"] / max(oc["bounce_queries"], 1)
+        node_s = oc["nodes_shadow"] / max(oc["shadow_queries"], 1)
+        tri_s = oc["prims_shadow"] / max(oc["shadow_queries"], 1)
+        alg_bytes = (counters["bounce_queries"] * (BYTES_PER_NODE * node_b + BYTES_PER_TRI * tri_b)
+                     + counters["shadow_queries"] * (BYTES_PER_NODE * node_s + BYTES_PER_TRI * tri_s)
+                     + counters["contributions"] * BYTES_PER_CONTRIB)
+        achieved = alg_bytes / avg_kernel_s / 1e9
+        roofline.update(achieved=round(achieved, 1), frac=round(achieved / HBM_PEAK_GBS, 4))
+        roofline["algorithmic_bytes_per_launch"] = int(alg_bytes)
+        roofline["per_query"] = {"canonical_nodes_bounce": round(node_b, 3), "canonical_tris_bounce": round(tri_b, 3),
+                                 "canonical_nodes_shadow": round(node_s, 3), "canonical_tris_shadow": round(tri_s, 3)}
+        result["cpu_baseline"] = {
+            "value": round((sample / N) / cpu_s, 5),
+            "unit": "RIRs/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"rays [0, {sample}) of the same {N}-ray trace (binary64 oracle, OpenMP over rays, "
+                      f"{cpu_s:.2f} s); CPU {cpu_model()}",
+            "primary_rays_per_s": round(sample / cpu_s, 1),
+        }
+        result["speedup_vs_cpu"] = round(result["value"] / result["cpu_baseline"]["value"], 2)
+        if sample == N:
+            result["accuracy_vs_fp64_oracle"] = band_errors(gpu_echogram, ref["echogram"], bin_dur)
+            tot_g, tot_o = gpu_echogram.sum(axis=1), ref["echogram"].sum(axis=1)
+            result["accuracy_vs_fp64_oracle"]["band_energy_max_rel_err"] = float(np.max(np.abs(tot_g - tot_o) / tot_o))
+    traffic_file = REPO / "profiles" / "trace_kernel_traffic.json"
+    if traffic_file.exists():
+        try:
+            roofline["traffic"] = json.loads(traffic_file.read_text()).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            pass
+    result["roofline"] = roofline
+    print(json.dumps(result), flush=True)
+    scene.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -151,6 +151,44 @@ int aa_trace(aa_scene_t scene, const aa_trace_params *params, const float *d_sou
 /* Synchronise `stream` and report a kernel-side overflow flag, if any. */
 int aa_stream_sync(void *stream);
 
+/* ---- persistent batches (replaces the thread-pool fan-out of one trace per
+ *      call: simulate_batch acoustic_agent/batch.py:139-184, run_many /
+ *      run_batch / run_sources api.py:477-639, and the per-channel traces of
+ *      engine.py:64-73) ----
+ * One submission traces `count` independent traces (any mix of scenes, ray
+ * counts, bounce limits, bins and source sets) in one persistent launch that
+ * pulls packets of `packet_rays` rays from a device work queue.  Each trace's
+ * outputs are zeroed by the submission.  The tuning fields of each trace's
+ * aa_trace_params (block_size, lds_*) are ignored; the batch tuning applies.
+ * A batch object owns a pinned descriptor table: a new submission on the same
+ * object waits until the previous one has finished reading it. */
+typedef struct aa_batch_s *aa_batch_t;
+
+typedef struct {
+    aa_scene_t scene;
+    aa_trace_params params;
+    const float *d_sources;          /* [Ks,4] */
+    const float *d_directions;       /* AA_RNG_ARRAYS only */
+    const float *d_diffuse_bank;     /* [M,3] */
+    const float *d_diffuse_random;   /* AA_RNG_ARRAYS only */
+    const int32_t *d_diffuse_indices;/* AA_RNG_ARRAYS only */
+    aa_trace_outputs outputs;
+} aa_batch_trace;
+
+typedef struct {
+    int32_t block_size;         /* 64, 128, 256 or 512 lanes per workgroup */
+    int32_t packet_rays;        /* rays per work item */
+    int32_t lds_nodes;          /* top 4-wide nodes staged in LDS per scene */
+    int32_t lds_histogram;      /* 1: privatise each item's histogram in LDS */
+    int32_t workgroups_per_cu;  /* persistent grid = CUs x this (0: from LDS use) */
+    int32_t use_graph;          /* 1: capture copy + zero + kernel in a hipGraph and replay it */
+} aa_batch_tuning;
+
+int aa_batch_create(int32_t device, int32_t max_traces, aa_batch_t *out);
+int aa_batch_submit(aa_batch_t batch, const aa_batch_trace *traces, int32_t count, const aa_batch_tuning *tuning,
+                    void *stream);
+int aa_batch_release(aa_batch_t batch);
+
 #ifdef __cplusplus
 }
 #endif

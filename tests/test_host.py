@@ -1,0 +1,194 @@
+"""Host logic on the CPU: the C ABI library, dispatch rules, scene compiler."""
+from __future__ import annotations
+
+import ctypes
+import math
+import re
+from dataclasses import replace
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from acoustic_agent_amd import _lib, hip_backend, scenes
+from acoustic_agent_amd.models import SimConfig
+from acoustic_agent_amd.scene import BoxSpec, Material, RoomScene, RoomSpec, WallSpec, tessellate
+from acoustic_agent_amd.steam_rt import trace_energy_field
+
+REPO = Path(__file__).resolve().parents[1]
+
+
+def _declared_symbols() -> set[str]:
+    names = set()
+    for header in (REPO / "include").glob("*.h"):
+        text = header.read_text()
+        names.update(re.findall(r"^\s*(?:int|const char \*)\s*(aa_\w+)\s*\(", text, flags=re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    declared = _declared_symbols()
+    assert {"aa_trace", "aa_scene_create", "aa_build_bvh", "aa_device_count"} <= declared
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(_lib.exported_symbols()) <= declared
+
+
+def test_no_device_means_not_available():
+    # this container has no GPU: the probe answers False instead of raising
+    assert hip_backend.hip_available(0) in (False, True)
+    assert hip_backend.hip_available(10_000) is False
+
+
+def test_abi_rejects_bad_inputs_before_touching_the_device():
+    lib = _lib.load()
+    v = np.zeros((1, 9), dtype=np.float32)
+    v[0, 3:6] = (1, 0, 0)
+    v[0, 6:9] = (0, 1, 0)
+    n = np.zeros((1, 3), dtype=np.float32)
+    refl = np.ones((1, 6), dtype=np.float32)
+    scat = np.zeros(1, dtype=np.float32)
+    handle = ctypes.c_void_p()
+
+    def create(surface_id, leaf=4, count=1):
+        s = np.asarray([surface_id], dtype=np.int32)
+        return lib.aa_scene_create(0, v.ctypes.data, s.ctypes.data, n.ctypes.data, count, refl.ctypes.data,
+                                   scat.ctypes.data, 1, leaf, ctypes.byref(handle))
+
+    assert create(5) == _lib.AA_ERR_INVALID
+    assert "surface id" in lib.aa_last_error().decode()
+    assert create(0, leaf=9) == _lib.AA_ERR_INVALID
+    assert create(0, count=0) == _lib.AA_ERR_INVALID
+    with pytest.raises(ValueError):
+        _lib.check(create(-1))
+    params = _lib.TraceParams()
+    out = _lib.TraceOutputs()
+    assert lib.aa_trace(None, ctypes.byref(params), None, None, None, None, None, ctypes.byref(out), None) == \
+        _lib.AA_ERR_INVALID
+
+
+# ---------------------------------------------------------------------------
+# dispatch (steam_rt.py:1757-1819 and tests/test_accelerators.py:117-155)
+# ---------------------------------------------------------------------------
+
+
+@pytest.fixture(scope="module")
+def small():
+    w = scenes.shoebox_workload(0)
+    cfg = SimConfig(duration_s=0.3, rt_duration_s=0.3, rt_num_rays=256, rt_num_bounces=6)
+    return w, cfg
+
+
+@pytest.mark.parametrize("field,value", [("rt_accelerator", "metal"), ("rt_precision", "float16")])
+def test_invalid_accelerator_configuration_is_rejected(small, field, value):
+    w, cfg = small
+    with pytest.raises(ValueError):
+        trace_energy_field(w.scene, w.source, w.listener, replace(cfg, **{field: value}))
+
+
+def test_reference_backends_are_not_shipped(small):
+    w, cfg = small
+    with pytest.raises(RuntimeError, match="reference engine"):
+        trace_energy_field(w.scene, w.source, w.listener, replace(cfg, rt_accelerator="numba"))
+
+
+def test_hip_rejects_fp64_before_launch(small, monkeypatch):
+    w, cfg = small
+    monkeypatch.setattr(hip_backend, "hip_available", lambda _device_id=0: True)
+    with pytest.raises(ValueError, match="float32"):
+        trace_energy_field(w.scene, w.source, w.listener, replace(cfg, rt_precision="float64"))
+
+
+def test_hip_without_device_raises(small, monkeypatch):
+    w, cfg = small
+    monkeypatch.setattr(hip_backend, "hip_available", lambda _device_id=0: False)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        trace_energy_field(w.scene, w.source, w.listener, cfg)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        trace_energy_field(w.scene, w.source, w.listener, replace(cfg, rt_accelerator="auto"))
+
+
+# ---------------------------------------------------------------------------
+# scene compiler
+# ---------------------------------------------------------------------------
+
+_MAT = Material(absorption=(0.1, 0.2, 0.3, 0.1, 0.2, 0.3), scattering=0.15)
+
+
+def _areas(mesh) -> np.ndarray:
+    v = mesh.vertices.reshape(-1, 3, 3)
+    a = 0.5 * np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
+    return np.bincount(mesh.surface, weights=a, minlength=mesh.num_surfaces)
+
+
+def test_surface_order_and_arrays_follow_the_reference():
+    spec = RoomSpec(corners=((0, 0), (4, 0), (4, 3), (0, 3)), height=2.5, wall_material=_MAT,
+                    boxes=(BoxSpec(center=(2, 1.5), size=(1, 0.5, 0.8), rotation_deg=30, material=_MAT),
+                           BoxSpec(center=(9, 9), size=(1, 1, 1), material=_MAT)))
+    room = RoomScene(spec)
+    a = room.kernel_arrays()
+    assert list(a["kinds"]) == [0, 0, 0, 0, 1, 1, 2]  # the box outside the outline is dropped
+    assert a["names"][4:6] == ("floor", "ceiling")
+    np.testing.assert_allclose(a["reflection"], 1.0 - a["absorption"])
+    np.testing.assert_allclose(a["scattering"], 0.15)
+    np.testing.assert_allclose(a["normals"][0], (0, -1, 0))  # wall (0,0)->(4,0): (dy, -dx)/|d|
+    np.testing.assert_allclose(a["box_z"][6], (0.0, 0.8))
+    lo, hi = room.surface_bounds()
+    np.testing.assert_allclose(lo[0], (-2e-6, -2e-6, -2e-6))
+    np.testing.assert_allclose(hi[4], (4 + 2e-6, 3 + 2e-6, 2e-6))
+
+
+def test_tessellation_preserves_surface_area_and_normals():
+    w = scenes.l_plan_workload(seed=2, target_triangles=4000)
+    a = w.scene.kernel_arrays()
+    area = _areas(w.mesh)
+    for si in range(w.scene.num_surfaces):
+        kind = int(a["kinds"][si])
+        if kind == 0:
+            expect = np.linalg.norm(a["wall_delta"][si]) * (a["wall_z"][si][1] - a["wall_z"][si][0])
+        elif kind == 1:
+            c = a["corners"]
+            expect = 0.5 * abs(np.sum(c[:, 0] * np.roll(c[:, 1], -1) - np.roll(c[:, 0], -1) * c[:, 1]))
+        else:
+            hw, hd = a["box_half"][si]
+            h = a["box_z"][si][1] - a["box_z"][si][0]
+            expect = 2 * (4 * hw * hd) + 2 * (2 * hw * h) + 2 * (2 * hd * h)
+        assert area[si] == pytest.approx(expect, rel=1e-9), si
+    v = w.mesh.vertices.reshape(-1, 3, 3)
+    g = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    g /= np.linalg.norm(g, axis=1, keepdims=True)
+    np.testing.assert_allclose(np.abs(np.sum(g * w.mesh.normals, axis=1)), 1.0, atol=1e-9)
+    edges = np.linalg.norm(np.roll(v, -1, axis=1) - v, axis=2).max()
+    assert edges <= w.max_edge * (1 + 1e-9)
+
+
+def test_tessellate_to_count_lands_near_target():
+    w = scenes.l_plan_workload(seed=0, target_triangles=20000)
+    assert abs(w.mesh.num_triangles - 20000) / 20000 < 0.25
+    assert w.scene.num_surfaces == 6 + 6 + 2 + 60
+
+
+def test_random_scene_covers_the_triangle_range():
+    small = scenes.random_scene(1, 12)
+    assert small.mesh.num_triangles == 12
+    big = scenes.random_scene(2, 30000)
+    assert 20000 < big.mesh.num_triangles < 45000
+
+
+def test_lintel_walls_span_the_door_top():
+    room = scenes.l_plan(0)
+    lintels = [s for s in room.surfaces if s["name"].endswith("lintel")]
+    assert len(lintels) == 2
+    for s in lintels:
+        assert s["z"] == (2.1, 2.8)
+        assert math.isclose(np.linalg.norm(s["b"] - s["a"]), 0.8)
+
+
+def test_wall_spec_clamps_to_room_height():
+    spec = RoomSpec(corners=((0, 0), (4, 0), (4, 3), (0, 3)), height=2.5, wall_material=_MAT,
+                    walls=(WallSpec((0, 0), (4, 0), z_min=-1.0, z_max=9.0),))
+    room = RoomScene(spec)
+    assert room.surfaces[0]["z"] == (0.0, 2.5)
+    mesh = tessellate(room.kernel_arrays())
+    assert mesh.num_triangles == 2 + 2 + 2
