@@ -24,7 +24,7 @@ AA_RNG_ARRAYS = 0
 AA_RNG_COUNTER = 1
 AA_NUM_BANDS = 6
 AA_MAX_SOURCES = 64
-AA_CNT_COUNT = 8
+AA_CNT_COUNT = 9
 
 COUNTER_NAMES = (
     "bounce_queries",
@@ -35,6 +35,7 @@ COUNTER_NAMES = (
     "triangles_shadow",
     "contributions",
     "escaped_rays",
+    "rays_claimed",
 )
 
 
@@ -78,6 +79,7 @@ class TraceParams(ctypes.Structure):
         ("block_size", ctypes.c_int32),
         ("lds_histogram", ctypes.c_int32),
         ("lds_nodes", ctypes.c_int32),
+        ("workgroups_per_cu", ctypes.c_int32),
         ("log_rays", ctypes.c_int32),
         ("log_bounces", ctypes.c_int32),
     ]

@@ -46,7 +46,6 @@ struct BatchTrace {
     int first_packet;  // prefix sum of packets over the table
     int num_packets;
     int lds_surf;      // single source and S small enough for LDS tallies
-    int hist_words;    // Ks * 6 * K
     unsigned* zero_ptr[8];
     unsigned zero_words[8];
 };
@@ -70,31 +69,36 @@ __global__ void batch_zero_kernel(const BatchHeader* hdr, const BatchTrace* trac
     }
 }
 
-template <int WG, bool LDS_HIST>
+// Work items are claimed per wave, so that a wave whose rays run long does
+// not hold up the other waves of its workgroup; each wave has its own slot of
+// staged BVH nodes, its own surface tallies and its own ray cursor in LDS.
+template <int WG>
 __global__ __launch_bounds__(WG) void batch_kernel(BatchHeader* hdr, const BatchTrace* traces, int lds_nodes_cap,
-                                                   int hist_cap, int surf_cap) {
+                                                   int surf_cap) {
+    constexpr int kWaves = WG / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned short* s_stack = reinterpret_cast<unsigned short*>(smem);
     float4* s_nodes = reinterpret_cast<float4*>(smem + kStackBytes<WG>());
-    float* s_hist = reinterpret_cast<float*>(s_nodes + (size_t)lds_nodes_cap * 8);
-    int* s_hits = reinterpret_cast<int*>(s_hist + (LDS_HIST ? hist_cap : 0));
-    int* s_contrib = s_hits + surf_cap;
-    float* s_senergy = reinterpret_cast<float*>(s_contrib + surf_cap);
-    __shared__ int s_item;
-    __shared__ int s_next_ray;
+    int* s_surf = reinterpret_cast<int*>(s_nodes + (size_t)kWaves * lds_nodes_cap * 8);
+    __shared__ unsigned s_cursor[kWaves];
 
-    const int tid = threadIdx.x;
+    const int wave = (int)(threadIdx.x >> 6);
+    const int lane = (int)(threadIdx.x & 63);
+    float4* my_nodes = s_nodes + (size_t)wave * lds_nodes_cap * 8;
+    int* my_hits = s_surf + (size_t)wave * surf_cap * 3;
+    int* my_contrib = my_hits + surf_cap;
+    float* my_energy = reinterpret_cast<float*>(my_contrib + surf_cap);
     const int num_traces = hdr->num_traces;
     const int total = hdr->total_packets;
     const int packet_rays = hdr->packet_rays;
-    int cur = -1;
+    const float4* staged = nullptr;  // scene whose top nodes sit in my_nodes
+    int staged_count = 0;
     while (true) {
-        if (tid == 0) s_item = (int)atomicAdd(&hdr->queue_head, 1u);
-        __syncthreads();
-        const int item = s_item;
-        if (item >= total) break;  // uniform: every workgroup exits once the queue is drained
-        // trace of this item: last t with first_packet <= item
-        int lo = 0, hi = num_traces - 1;
+        unsigned claimed = 0u;
+        if (lane == 0) claimed = atomicAdd(&hdr->queue_head, 1u);
+        const int item = (int)__shfl(claimed, 0);
+        if (item >= total) break;  // wave-uniform: every wave exits once the queue is drained
+        int lo = 0, hi = num_traces - 1;  // last trace with first_packet <= item
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (traces[mid].first_packet <= item)
@@ -102,60 +106,48 @@ __global__ __launch_bounds__(WG) void batch_kernel(BatchHeader* hdr, const Batch
             else
                 hi = mid - 1;
         }
-        const int t = lo;
-        const BatchTrace& bt = traces[t];
+        const BatchTrace& bt = traces[lo];
         SceneView sv = bt.sv;
         sv.lds_nodes = min(lds_nodes_cap, sv.use_bvh ? sv.num_nodes : 0);
-        if (t != cur) {
-            for (int i = tid; i < sv.lds_nodes * 8; i += WG) s_nodes[i] = sv.nodes[i];
-            cur = t;
+        if (sv.nodes != staged || sv.lds_nodes != staged_count) {
+            for (int i = lane; i < sv.lds_nodes * 8; i += 64) my_nodes[i] = sv.nodes[i];
+            staged = sv.nodes;
+            staged_count = sv.lds_nodes;
         }
-        const bool lds_hist = LDS_HIST && bt.hist_words <= hist_cap;
-        if (lds_hist)
-            for (int i = tid; i < bt.hist_words; i += WG) s_hist[i] = 0.0f;
-        if (bt.lds_surf) {
-            for (int i = tid; i < sv.num_surfaces; i += WG) {
-                s_hits[i] = 0;
-                s_contrib[i] = 0;
-                s_senergy[i] = 0.0f;
+        const bool lds_surf = bt.lds_surf != 0;
+        if (lds_surf) {
+            for (int i = lane; i < sv.num_surfaces; i += 64) {
+                my_hits[i] = 0;
+                my_contrib[i] = 0;
+                my_energy[i] = 0.0f;
             }
         }
+        if (lane == 0) s_cursor[wave] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         const int begin = (item - bt.first_packet) * packet_rays;
-        const int end = min(begin + packet_rays, bt.p.num_rays);
-        if (tid == 0) s_next_ray = begin;
-        __syncthreads();
+        const int count = min(packet_rays, bt.p.num_rays - begin);
         Lds lds;
         lds.stack = s_stack;
-        lds.nodes = s_nodes;
-        lds.hist = s_hist;
-        lds.hits = bt.lds_surf ? s_hits : nullptr;
-        lds.contrib = bt.lds_surf ? s_contrib : nullptr;
-        lds.senergy = bt.lds_surf ? s_senergy : nullptr;
+        lds.nodes = my_nodes;
+        lds.hist = nullptr;
+        lds.hits = lds_surf ? my_hits : nullptr;
+        lds.contrib = lds_surf ? my_contrib : nullptr;
+        lds.senergy = lds_surf ? my_energy : nullptr;
         LaneStats st;
-        while (true) {
-            const int ray = atomicAdd(&s_next_ray, 1);
-            if (ray >= end) break;
-            if (lds_hist)
-                trace_one_ray<WG, true>(sv, bt.p, bt.out, lds, ray, st, bt.error_flag);
-            else
-                trace_one_ray<WG, false>(sv, bt.p, bt.out, lds, ray, st, bt.error_flag);
-        }
+        trace_rays_refill<WG, false>(sv, bt.p, bt.out, lds, &s_cursor[wave], begin, count, st, bt.error_flag);
         flush_stats(st, bt.out, bt.p.num_sources);
-        __syncthreads();
-        if (lds_hist) {
-            for (int i = tid; i < bt.hist_words; i += WG) {
-                const float v = s_hist[i];
-                if (v != 0.0f) atomicAdd(&bt.out.echogram[i], v);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lds_surf) {
+            for (int i = lane; i < sv.num_surfaces; i += 64) {
+                if (my_hits[i]) atomicAdd(&bt.out.hit_counts[i], my_hits[i]);
+                if (my_contrib[i]) atomicAdd(&bt.out.contrib_counts[i], my_contrib[i]);
+                if (my_energy[i] != 0.0f) atomicAdd(&bt.out.surface_energy[i], my_energy[i]);
             }
         }
-        if (bt.lds_surf) {
-            for (int i = tid; i < sv.num_surfaces; i += WG) {
-                if (s_hits[i]) atomicAdd(&bt.out.hit_counts[i], s_hits[i]);
-                if (s_contrib[i]) atomicAdd(&bt.out.contrib_counts[i], s_contrib[i]);
-                if (s_senergy[i] != 0.0f) atomicAdd(&bt.out.surface_energy[i], s_senergy[i]);
-            }
-        }
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -181,16 +173,24 @@ size_t table_size(int max_traces) {
     return hdr + (size_t)max_traces * sizeof(aa::BatchTrace);
 }
 
-template <int WG, bool LDS>
+template <int WG>
 hipError_t launch_batch(aa::BatchHeader* hdr, const aa::BatchTrace* traces, int grid, size_t lds, int lds_nodes,
-                        int hist_cap, int surf_cap, hipStream_t stream) {
-    auto kern = aa::batch_kernel<WG, LDS>;
+                        int surf_cap, hipStream_t stream) {
+    auto kern = aa::batch_kernel<WG>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), lds, stream, hdr, traces, lds_nodes, hist_cap, surf_cap);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), lds, stream, hdr, traces, lds_nodes, surf_cap);
     return hipGetLastError();
+}
+
+template <int WG>
+int occupancy_blocks(size_t lds) {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)aa::batch_kernel<WG>, WG, lds) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
 }
 
 }  // namespace
@@ -257,26 +257,21 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
     const aa::DeviceProps& props = aa::device_props(b->device);
     if (!props.valid) return fail(AA_ERR_DEVICE, "cannot query device %d", b->device);
 
-    // LDS plan shared by every trace of the launch
-    int max_nodes = 0, max_hist = 0, max_surf = 0;
+    // LDS plan shared by every trace of the launch: per-wave node slots and
+    // per-wave surface tallies (histograms go to global atomics)
+    int max_nodes = 0, max_surf = 0;
     for (int32_t i = 0; i < count; ++i) {
         const aa_batch_trace& t = traces[i];
         if (t.params.use_bvh) max_nodes = std::max(max_nodes, (int)t.scene->wide.nodes.size());
-        max_hist = std::max(max_hist, t.params.num_sources * AA_NUM_BANDS * t.params.num_bins);
         if (t.params.num_sources == 1) max_surf = std::max(max_surf, t.scene->num_surfaces);
     }
+    const int waves = wg / 64;
     const int lds_nodes = std::min(std::max(tuning->lds_nodes, 0), max_nodes);
     const size_t lds_cap = props.lds_per_block > 0 ? props.lds_per_block : 65536;
-    const size_t base = aa::stack_bytes(wg) + (size_t)lds_nodes * sizeof(aa::WideNode);
-    const int surf_cap = (max_surf > 0 && base + (size_t)max_surf * 12 <= std::min<size_t>(lds_cap, 96 * 1024))
-                             ? max_surf : 0;
-    int hist_cap = 0;
-    if (tuning->lds_histogram > 0) {
-        const size_t room = lds_cap > base + (size_t)surf_cap * 12 ? lds_cap - base - (size_t)surf_cap * 12 : 0;
-        hist_cap = (int)std::min<size_t>((size_t)max_hist, room / 4);
-    }
-    const bool lds_hist = hist_cap > 0;
-    const size_t lds = base + (size_t)(lds_hist ? hist_cap : 0) * 4 + (size_t)surf_cap * 12;
+    const size_t base = aa::stack_bytes(wg) + (size_t)waves * lds_nodes * sizeof(aa::WideNode);
+    const size_t surf_budget = 32 * 1024;
+    const int surf_cap = (max_surf > 0 && (size_t)waves * max_surf * 12 <= surf_budget) ? max_surf : 0;
+    const size_t lds = base + (size_t)waves * surf_cap * 12;
     if (lds > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", lds, lds_cap);
 
     // the pinned table may still be read by the previous submission's copy
@@ -297,7 +292,6 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
         d.num_packets = (t.params.num_rays + packet_rays - 1) / packet_rays;
         d.first_packet = (int)packets;
         packets += d.num_packets;
-        d.hist_words = t.params.num_sources * AA_NUM_BANDS * t.params.num_bins;
         d.lds_surf = (t.params.num_sources == 1 && t.scene->num_surfaces <= surf_cap) ? 1 : 0;
         const aa::OutputExtents ext = aa::output_extents(&t.outputs, &t.params, t.scene->num_surfaces);
         for (int r = 0; r < 8; ++r) {
@@ -313,10 +307,18 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
 
     aa::BatchHeader* d_hdr = static_cast<aa::BatchHeader*>(b->d_table);
     const aa::BatchTrace* d_tr = reinterpret_cast<const aa::BatchTrace*>(static_cast<char*>(b->d_table) + hdr_bytes);
-    const int per_cu = tuning->workgroups_per_cu > 0 ? tuning->workgroups_per_cu
-                                                     : std::max(1, (int)(lds_cap / std::max<size_t>(lds, 1)));
-    const int grid = std::max(1, std::min<int>(props.cu * per_cu, (int)packets));
-    const int persistent_grid = tuning->use_graph ? std::max(1, props.cu * per_cu) : grid;
+    int per_cu = tuning->workgroups_per_cu;
+    if (per_cu <= 0) {
+        switch (wg) {
+            case 64: per_cu = occupancy_blocks<64>(lds); break;
+            case 128: per_cu = occupancy_blocks<128>(lds); break;
+            case 256: per_cu = occupancy_blocks<256>(lds); break;
+            default: per_cu = occupancy_blocks<512>(lds); break;
+        }
+    }
+    const int wave_items = (int)std::min<int64_t>(packets, (int64_t)1 << 30);
+    const int grid = std::max(1, std::min(props.cu * per_cu, (wave_items + waves - 1) / waves));
+    const int persistent_grid = std::max(1, props.cu * per_cu);
 
     auto enqueue = [&](hipStream_t s) -> hipError_t {
         hipError_t e = hipMemcpyAsync(b->d_table, b->h_table,
@@ -328,10 +330,9 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
         e = hipGetLastError();
         if (e != hipSuccess) return e;
         const int g = tuning->use_graph ? persistent_grid : grid;
-#define AA_LAUNCH(W)                                                                                       \
-    case W:                                                                                                \
-        e = lds_hist ? launch_batch<W, true>(d_hdr, d_tr, g, lds, lds_nodes, hist_cap, surf_cap, s)       \
-                     : launch_batch<W, false>(d_hdr, d_tr, g, lds, lds_nodes, hist_cap, surf_cap, s);     \
+#define AA_LAUNCH(W)                                                             \
+    case W:                                                                      \
+        e = launch_batch<W>(d_hdr, d_tr, g, lds, lds_nodes, surf_cap, s);        \
         break;
         switch (wg) {
             AA_LAUNCH(64)
@@ -344,7 +345,7 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
     };
 
     if (tuning->use_graph) {
-        const auto key = std::make_tuple(wg, lds_nodes, hist_cap, surf_cap, persistent_grid, (int)lds_hist, stream);
+        const auto key = std::make_tuple(wg, lds_nodes, 0, surf_cap, persistent_grid, 0, stream);
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
             hipStream_t cap;

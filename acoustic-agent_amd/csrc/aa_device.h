@@ -398,221 +398,310 @@ __device__ __forceinline__ void initial_direction(const RayParams& p, int ray, f
     dz = (float)z;
 }
 
-// Trace one listener ray through its bounces and deposit its next-event
-// contributions.  LDS_HIST: histogram privatised in lds.hist, else global
-// atomics into out.echogram.  lds.hits != null: single-source per-surface
-// tallies privatised in LDS.
-template <int WG, bool LDS_HIST>
-__device__ void trace_one_ray(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds, int ray,
-                              LaneStats& st, int* error_flag) {
-    unsigned short* stk = lds.stack + threadIdx.x;
-    Ray r;
-    r.ox = p.lx;
-    r.oy = p.ly;
-    r.oz = p.lz;
-    initial_direction(p, ray, r.dx, r.dy, r.dz);
-    const float d0x = r.dx, d0y = r.dy, d0z = r.dz;
-    float A[kBands];
-#pragma unroll
-    for (int b = 0; b < kBands; ++b) A[b] = 1.0f;
-    float accum = 0.0f;
-    const int K = p.num_bins;
-    const int Ks = p.num_sources;
-    const unsigned long long full_mask = Ks >= 64 ? ~0ull : ((1ull << Ks) - 1ull);
-    unsigned long long mask = full_mask;
-    bool alive = true;
-    int iter = 0;
-    const bool logging = out.log_tri != nullptr && ray < out.log_rays;
-    for (int bounce = 0; bounce < p.num_bounces; ++bounce) {
-        iter = bounce + 1;
-        r.ix = safe_inv(r.dx);
-        r.iy = safe_inv(r.dy);
-        r.iz = safe_inv(r.dz);
-        Closest h;
-        ++st.q_b;
-        closest_hit<WG>(sv, lds.nodes, stk, r, h, st.nb, st.tb, error_flag);
-        if (logging && bounce < out.log_bounces) {
-            out.log_tri[(size_t)ray * out.log_bounces + bounce] = h.tri;
-            out.log_t[(size_t)ray * out.log_bounces + bounce] = h.t;
-        }
-        if (h.surf < 0) ++st.escaped;
-        if (h.surf < 0 || h.t <= p.listener_radius || accum > p.max_path_len) {
-            alive = false;
-            break;
-        }
-        if (bounce > 0) {
-            if (sphere_before(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, p.lx, p.ly, p.lz, p.listener_radius, h.t)) {
-                alive = false;
-                break;
-            }
-            unsigned long long m = mask;
-            while (m) {
-                const int k = __builtin_ctzll(m);
-                m &= m - 1ull;
-                const float4 S = p.sources[k];
-                const float sdist =
-                    sqrtf((p.lx - S.x) * (p.lx - S.x) + (p.ly - S.y) * (p.ly - S.y) + (p.lz - S.z) * (p.lz - S.z));
-                if (sdist > p.source_radius &&
-                    sphere_before(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, S.x, S.y, S.z, p.source_radius, h.t)) {
-                    mask &= ~(1ull << k);
-                    atomicMax(&out.actual_bounces[k], bounce + 1);
-                }
-            }
-            if (mask == 0ull) {
-                alive = false;
-                break;
-            }
-        }
-        const float4 nn = sv.normals[h.tri];
-        float nx = nn.x, ny = nn.y, nz = nn.z;
-        if (nx * r.dx + ny * r.dy + nz * r.dz > 0.0f) {
-            nx = -nx;
-            ny = -ny;
-            nz = -nz;
-        }
-        const int surf = h.surf;
-        const float t = h.t;
-        const float hx = r.ox + t * r.dx + kHitOffset * nx;
-        const float hy = r.oy + t * r.dy + kHitOffset * ny;
-        const float hz = r.oz + t * r.dz + kHitOffset * nz;
-        const float scat = sv.scattering[surf];
-        float refl[kBands];
-#pragma unroll
-        for (int b = 0; b < kBands; ++b) refl[b] = sv.reflection[surf * kBands + b];
-        unsigned long long m = mask;
-        while (m) {
-            const int k = __builtin_ctzll(m);
-            m &= m - 1ull;
-            if (lds.hits)
-                atomicAdd(&lds.hits[surf], 1);
-            else
-                atomicAdd(&out.hit_counts[(size_t)k * sv.num_surfaces + surf], 1);
-            const float4 S = p.sources[k];
-            const float tsx = S.x - hx, tsy = S.y - hy, tsz = S.z - hz;
-            const float D = sqrtf(tsx * tsx + tsy * tsy + tsz * tsz);
-            if (!(D > p.dmin)) continue;
-            const float sdx = tsx / D, sdy = tsy / D, sdz = tsz / D;
-            const float facing = nx * sdx + ny * sdy + nz * sdz;
-            if (!(facing > 0.0f)) continue;
-            Ray sr;
-            sr.ox = hx;
-            sr.oy = hy;
-            sr.oz = hz;
-            sr.dx = sdx;
-            sr.dy = sdy;
-            sr.dz = sdz;
-            sr.ix = safe_inv(sdx);
-            sr.iy = safe_inv(sdy);
-            sr.iz = safe_inv(sdz);
-            ++st.q_s;
-            if (any_hit<WG>(sv, lds.nodes, stk, sr, D, st.ns, st.ts, error_flag)) continue;
-            const float diffuse = 0.31830988618379067f * scat * facing;
-            float hvx = sdx - r.dx, hvy = sdy - r.dy, hvz = sdz - r.dz;
-            normalize3(hvx, hvy, hvz);
-            float cos_half = hvx * nx + hvy * ny + hvz * nz;
-            cos_half = cos_half < 0.0f ? 0.0f : cos_half;
-            const float specular =
-                (p.spec_exp + 2.0f) * 0.039788735772973836f * (1.0f - scat) * powf(cos_half, p.spec_exp);
-            const float dm = D > p.dmin ? D : p.dmin;
-            const float distance_term = 0.07957747154594767f / (dm * dm);
-            const float source_cosine = -(p.fx * sdx + p.fy * sdy + p.fz * sdz);
-            const float source_gain = powf(fabsf((1.0f - p.dip_w) + p.dip_w * source_cosine), p.dip_p);
-            const float rel_delay = (accum + t + D) / p.c - S.w;
-            const float q = rel_delay / p.bin_dur;
-            if (!(q >= 0.0f && q < (float)K)) continue;
-            const int bin = (int)floorf(q);
-            const float base = p.ray_scale * source_gain * distance_term * (diffuse + specular);
-            float esum = 0.0f;
-#pragma unroll
-            for (int b = 0; b < kBands; ++b) {
-                const float e = base * refl[b] * A[b];
-                const size_t hi = ((size_t)k * kBands + b) * K + bin;
-                if (LDS_HIST)
-                    atomicAdd(&lds.hist[hi], e);
-                else
-                    atomicAdd(&out.echogram[hi], e);
-                if (p.amb) {
-                    float* am = out.ambisonic + (((size_t)k * kBands + b) * 4) * K + bin;
-                    atomicAdd(am, e);
-                    atomicAdd(am + K, e * d0x);
-                    atomicAdd(am + 2 * K, e * d0y);
-                    atomicAdd(am + 3 * K, e * d0z);
-                }
-                esum += e;
-            }
-            ++st.contrib;
-            if (lds.hits) {
-                atomicAdd(&lds.contrib[surf], 1);
-                atomicAdd(&lds.senergy[surf], esum);
-            } else {
-                atomicAdd(&out.contrib_counts[(size_t)k * sv.num_surfaces + surf], 1);
-                atomicAdd(&out.surface_energy[(size_t)k * sv.num_surfaces + surf], esum);
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < kBands; ++b) A[b] *= refl[b];
-        accum += t;
-        r.ox = hx;
-        r.oy = hy;
-        r.oz = hz;
-        float u;
-        int j;
-        if (p.rng_mode == AA_RNG_ARRAYS) {
-            u = p.urand[(size_t)bounce * p.num_rays + ray];
-            j = p.uidx[(size_t)bounce * p.num_rays + ray];
-        } else {
-            unsigned w0, w1;
-            philox4x32_10((unsigned)bounce, (unsigned)ray, 0u, 0u, p.seed_lo, p.seed_hi, w0, w1);
-            u = (float)(w0 >> 8) * (1.0f / 16777216.0f);
-            j = (int)(((unsigned long long)w1 * (unsigned long long)(unsigned)p.num_samples) >> 32);
-        }
-        if (u < scat) {
-            const float sx = p.bank[j * 3], sy = p.bank[j * 3 + 1], sz = p.bank[j * 3 + 2];
-            float hx2 = 1.0f, hy2 = 0.0f, hz2 = 0.0f;
-            if (fabsf(nx) >= 0.9f) {
-                hx2 = 0.0f;
-                hy2 = 1.0f;
-            }
-            float tx = hy2 * nz - hz2 * ny;
-            float ty = hz2 * nx - hx2 * nz;
-            float tz = hx2 * ny - hy2 * nx;
-            normalize3(tx, ty, tz);
-            const float bx = ny * tz - nz * ty;
-            const float by = nz * tx - nx * tz;
-            const float bz = nx * ty - ny * tx;
-            float ndx = sx * tx + sy * bx + sz * nx;
-            float ndy = sx * ty + sy * by + sz * ny;
-            float ndz = sx * tz + sy * bz + sz * nz;
-            normalize3(ndx, ndy, ndz);
-            r.dx = ndx;
-            r.dy = ndy;
-            r.dz = ndz;
-        } else {
-            const float dot = r.dx * nx + r.dy * ny + r.dz * nz;
-            float ndx = r.dx - 2.0f * dot * nx;
-            float ndy = r.dy - 2.0f * dot * ny;
-            float ndz = r.dz - 2.0f * dot * nz;
-            normalize3(ndx, ndy, ndz);
-            r.dx = ndx;
-            r.dy = ndy;
-            r.dz = ndz;
-        }
-        if (accum > p.max_path_len) {
-            alive = false;
-            break;
-        }
-    }
-    if (mask == full_mask) {
+// Resumable state of one listener ray between bounces, so that a lane whose
+// ray has ended can start the next ray while the rest of its wave continues.
+struct RayState {
+    float ox, oy, oz, dx, dy, dz;
+    float d0x, d0y, d0z;  // initial direction (FOA weights)
+    float A[kBands];      // accumulated energy reflection per band
+    float accum;          // path length so far
+    unsigned long long mask;  // sources for which the ray is still alive
+    int ray;
+    int bounce;
+};
+
+__device__ __forceinline__ unsigned long long full_source_mask(int Ks) {
+    return Ks >= 64 ? ~0ull : ((1ull << Ks) - 1ull);
+}
+
+// End-of-ray bookkeeping (steam_rt.py:4745-4746, 4848-4849): actual bounces
+// and the active count, per source.
+__device__ __forceinline__ void finish_ray(const RayParams& p, const Outputs& out, const RayState& s, bool alive,
+                                           int iter, LaneStats& st) {
+    if (s.mask == full_source_mask(p.num_sources)) {
         st.max_iter_full = max(st.max_iter_full, iter);
         if (alive) ++st.active_full;
     } else {
-        unsigned long long m = mask;
+        unsigned long long m = s.mask;
         while (m) {
             const int k = __builtin_ctzll(m);
             m &= m - 1ull;
             atomicMax(&out.actual_bounces[k], iter);
             if (alive) atomicAdd(&out.active_count[k], 1);
         }
+    }
+}
+
+// Start `ray`; returns false if it has no bounce to trace (num_bounces == 0).
+__device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out, int ray, RayState& s,
+                                          LaneStats& st) {
+    s.ray = ray;
+    s.bounce = 0;
+    s.ox = p.lx;
+    s.oy = p.ly;
+    s.oz = p.lz;
+    initial_direction(p, ray, s.dx, s.dy, s.dz);
+    s.d0x = s.dx;
+    s.d0y = s.dy;
+    s.d0z = s.dz;
+#pragma unroll
+    for (int b = 0; b < kBands; ++b) s.A[b] = 1.0f;
+    s.accum = 0.0f;
+    s.mask = full_source_mask(p.num_sources);
+    if (p.num_bounces <= 0) {
+        finish_ray(p, out, s, true, 0, st);
+        return false;
+    }
+    return true;
+}
+
+// One bounce of the reference loop (steam_rt.py:4743-4848): closest hit,
+// termination tests, next-event deposit per live source, continuation.
+// Returns true if the ray needs another bounce; otherwise it has been
+// finished.  LDS_HIST: histogram privatised in lds.hist, else global atomics
+// into out.echogram.  lds.hits != null: single-source per-surface tallies in
+// LDS.
+template <int WG, bool LDS_HIST>
+__device__ bool step_ray(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds, RayState& s,
+                         LaneStats& st, int* error_flag) {
+    unsigned short* stk = lds.stack + threadIdx.x;
+    const int bounce = s.bounce;
+    const int K = p.num_bins;
+    Ray r;
+    r.ox = s.ox;
+    r.oy = s.oy;
+    r.oz = s.oz;
+    r.dx = s.dx;
+    r.dy = s.dy;
+    r.dz = s.dz;
+    r.ix = safe_inv(r.dx);
+    r.iy = safe_inv(r.dy);
+    r.iz = safe_inv(r.dz);
+    Closest h;
+    ++st.q_b;
+    closest_hit<WG>(sv, lds.nodes, stk, r, h, st.nb, st.tb, error_flag);
+    if (out.log_tri != nullptr && s.ray < out.log_rays && bounce < out.log_bounces) {
+        out.log_tri[(size_t)s.ray * out.log_bounces + bounce] = h.tri;
+        out.log_t[(size_t)s.ray * out.log_bounces + bounce] = h.t;
+    }
+    if (h.surf < 0) ++st.escaped;
+    if (h.surf < 0 || h.t <= p.listener_radius || s.accum > p.max_path_len) {
+        finish_ray(p, out, s, false, bounce + 1, st);
+        return false;
+    }
+    if (bounce > 0) {
+        if (sphere_before(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, p.lx, p.ly, p.lz, p.listener_radius, h.t)) {
+            finish_ray(p, out, s, false, bounce + 1, st);
+            return false;
+        }
+        unsigned long long m = s.mask;
+        while (m) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1ull;
+            const float4 S = p.sources[k];
+            const float sdist =
+                sqrtf((p.lx - S.x) * (p.lx - S.x) + (p.ly - S.y) * (p.ly - S.y) + (p.lz - S.z) * (p.lz - S.z));
+            if (sdist > p.source_radius &&
+                sphere_before(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, S.x, S.y, S.z, p.source_radius, h.t)) {
+                s.mask &= ~(1ull << k);
+                atomicMax(&out.actual_bounces[k], bounce + 1);
+            }
+        }
+        if (s.mask == 0ull) return false;  // every source has recorded this ray's end
+    }
+    const float4 nn = sv.normals[h.tri];
+    float nx = nn.x, ny = nn.y, nz = nn.z;
+    if (nx * r.dx + ny * r.dy + nz * r.dz > 0.0f) {
+        nx = -nx;
+        ny = -ny;
+        nz = -nz;
+    }
+    const int surf = h.surf;
+    const float t = h.t;
+    const float hx = r.ox + t * r.dx + kHitOffset * nx;
+    const float hy = r.oy + t * r.dy + kHitOffset * ny;
+    const float hz = r.oz + t * r.dz + kHitOffset * nz;
+    const float scat = sv.scattering[surf];
+    float refl[kBands];
+#pragma unroll
+    for (int b = 0; b < kBands; ++b) refl[b] = sv.reflection[surf * kBands + b];
+    unsigned long long m = s.mask;
+    while (m) {
+        const int k = __builtin_ctzll(m);
+        m &= m - 1ull;
+        if (lds.hits)
+            atomicAdd(&lds.hits[surf], 1);
+        else
+            atomicAdd(&out.hit_counts[(size_t)k * sv.num_surfaces + surf], 1);
+        const float4 S = p.sources[k];
+        const float tsx = S.x - hx, tsy = S.y - hy, tsz = S.z - hz;
+        const float D = sqrtf(tsx * tsx + tsy * tsy + tsz * tsz);
+        if (!(D > p.dmin)) continue;
+        const float sdx = tsx / D, sdy = tsy / D, sdz = tsz / D;
+        const float facing = nx * sdx + ny * sdy + nz * sdz;
+        if (!(facing > 0.0f)) continue;
+        Ray sr;
+        sr.ox = hx;
+        sr.oy = hy;
+        sr.oz = hz;
+        sr.dx = sdx;
+        sr.dy = sdy;
+        sr.dz = sdz;
+        sr.ix = safe_inv(sdx);
+        sr.iy = safe_inv(sdy);
+        sr.iz = safe_inv(sdz);
+        ++st.q_s;
+        if (any_hit<WG>(sv, lds.nodes, stk, sr, D, st.ns, st.ts, error_flag)) continue;
+        const float diffuse = 0.31830988618379067f * scat * facing;
+        float hvx = sdx - r.dx, hvy = sdy - r.dy, hvz = sdz - r.dz;
+        normalize3(hvx, hvy, hvz);
+        float cos_half = hvx * nx + hvy * ny + hvz * nz;
+        cos_half = cos_half < 0.0f ? 0.0f : cos_half;
+        const float specular =
+            (p.spec_exp + 2.0f) * 0.039788735772973836f * (1.0f - scat) * powf(cos_half, p.spec_exp);
+        const float dm = D > p.dmin ? D : p.dmin;
+        const float distance_term = 0.07957747154594767f / (dm * dm);
+        const float source_cosine = -(p.fx * sdx + p.fy * sdy + p.fz * sdz);
+        const float source_gain = powf(fabsf((1.0f - p.dip_w) + p.dip_w * source_cosine), p.dip_p);
+        const float rel_delay = (s.accum + t + D) / p.c - S.w;
+        const float q = rel_delay / p.bin_dur;
+        if (!(q >= 0.0f && q < (float)K)) continue;
+        const int bin = (int)floorf(q);
+        const float base = p.ray_scale * source_gain * distance_term * (diffuse + specular);
+        float esum = 0.0f;
+#pragma unroll
+        for (int b = 0; b < kBands; ++b) {
+            const float e = base * refl[b] * s.A[b];
+            const size_t hi = ((size_t)k * kBands + b) * K + bin;
+            if (LDS_HIST)
+                atomicAdd(&lds.hist[hi], e);
+            else
+                atomicAdd(&out.echogram[hi], e);
+            if (p.amb) {
+                float* am = out.ambisonic + (((size_t)k * kBands + b) * 4) * K + bin;
+                atomicAdd(am, e);
+                atomicAdd(am + K, e * s.d0x);
+                atomicAdd(am + 2 * K, e * s.d0y);
+                atomicAdd(am + 3 * K, e * s.d0z);
+            }
+            esum += e;
+        }
+        ++st.contrib;
+        if (lds.hits) {
+            atomicAdd(&lds.contrib[surf], 1);
+            atomicAdd(&lds.senergy[surf], esum);
+        } else {
+            atomicAdd(&out.contrib_counts[(size_t)k * sv.num_surfaces + surf], 1);
+            atomicAdd(&out.surface_energy[(size_t)k * sv.num_surfaces + surf], esum);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < kBands; ++b) s.A[b] *= refl[b];
+    s.accum += t;
+    s.ox = hx;
+    s.oy = hy;
+    s.oz = hz;
+    float u;
+    int j;
+    if (p.rng_mode == AA_RNG_ARRAYS) {
+        u = p.urand[(size_t)bounce * p.num_rays + s.ray];
+        j = p.uidx[(size_t)bounce * p.num_rays + s.ray];
+    } else {
+        unsigned w0, w1;
+        philox4x32_10((unsigned)bounce, (unsigned)s.ray, 0u, 0u, p.seed_lo, p.seed_hi, w0, w1);
+        u = (float)(w0 >> 8) * (1.0f / 16777216.0f);
+        j = (int)(((unsigned long long)w1 * (unsigned long long)(unsigned)p.num_samples) >> 32);
+    }
+    if (u < scat) {
+        const float sx = p.bank[j * 3], sy = p.bank[j * 3 + 1], sz = p.bank[j * 3 + 2];
+        float hx2 = 1.0f, hy2 = 0.0f, hz2 = 0.0f;
+        if (fabsf(nx) >= 0.9f) {
+            hx2 = 0.0f;
+            hy2 = 1.0f;
+        }
+        float tx = hy2 * nz - hz2 * ny;
+        float ty = hz2 * nx - hx2 * nz;
+        float tz = hx2 * ny - hy2 * nx;
+        normalize3(tx, ty, tz);
+        const float bx = ny * tz - nz * ty;
+        const float by = nz * tx - nx * tz;
+        const float bz = nx * ty - ny * tx;
+        float ndx = sx * tx + sy * bx + sz * nx;
+        float ndy = sx * ty + sy * by + sz * ny;
+        float ndz = sx * tz + sy * bz + sz * nz;
+        normalize3(ndx, ndy, ndz);
+        s.dx = ndx;
+        s.dy = ndy;
+        s.dz = ndz;
+    } else {
+        const float dot = r.dx * nx + r.dy * ny + r.dz * nz;
+        float ndx = r.dx - 2.0f * dot * nx;
+        float ndy = r.dy - 2.0f * dot * ny;
+        float ndz = r.dz - 2.0f * dot * nz;
+        normalize3(ndx, ndy, ndz);
+        s.dx = ndx;
+        s.dy = ndy;
+        s.dz = ndz;
+    }
+    if (s.accum > p.max_path_len) {
+        finish_ray(p, out, s, false, bounce + 1, st);
+        return false;
+    }
+    s.bounce = bounce + 1;
+    if (s.bounce >= p.num_bounces) {
+        finish_ray(p, out, s, true, s.bounce, st);
+        return false;
+    }
+    return true;
+}
+
+// Hand out rays to the lanes of this wave that need one: one atomic per wave
+// on `cursor`; lanes past `limit` get -1.
+__device__ __forceinline__ int claim_rays(bool need, unsigned* cursor, int limit) {
+    const unsigned long long ballot = __ballot(need);
+    if (ballot == 0ull) return -1;
+    const int count = __popcll(ballot);
+    const int leader = __ffsll((long long)ballot) - 1;
+    const int lane = (int)(threadIdx.x & 63);
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(cursor, (unsigned)count);
+    base = __shfl(base, leader);
+    if (!need) return -1;
+    const unsigned long long below = ballot & ((1ull << lane) - 1ull);
+    const unsigned idx = base + (unsigned)__popcll(below);
+    return idx < (unsigned)limit ? (int)idx : -1;
+}
+
+// All rays of [first, first + count) handed out by `cursor` (relative to
+// `first`), traced by the calling wave's lanes with refill.
+template <int WG, bool LDS_HIST>
+__device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds,
+                                  unsigned* cursor, int first, int count, LaneStats& st, int* error_flag) {
+    RayState s;
+    bool has = false;
+    bool exhausted = false;  // wave-uniform: a claim came back empty, so the pool is drained
+    while (true) {
+        if (!exhausted) {
+            const bool need = !has;
+            const int got = claim_rays(need, cursor, count);
+            if (need && got >= 0) has = start_ray(p, out, first + got, s, st);
+            exhausted = __any(need && got < 0) != 0;
+        }
+        if (!__any(has)) {
+            if (exhausted) break;
+            continue;  // every lane's new ray had no bounce to trace; claim again
+        }
+        if (has) has = step_ray<WG, LDS_HIST>(sv, p, out, lds, s, st, error_flag);
+    }
+}
+
+// Trace one ray to the end (used where no refill is wanted).
+template <int WG, bool LDS_HIST>
+__device__ void trace_one_ray(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds, int ray,
+                              LaneStats& st, int* error_flag) {
+    RayState s;
+    if (!start_ray(p, out, ray, s, st)) return;
+    while (step_ray<WG, LDS_HIST>(sv, p, out, lds, s, st, error_flag)) {
     }
 }
 

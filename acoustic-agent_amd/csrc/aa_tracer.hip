@@ -72,7 +72,7 @@ __global__ void fill_zero_kernel(unsigned* p, size_t words) {
         p[i] = 0u;
 }
 
-// One lane per ray, grid-stride over the rays of one trace.
+// Persistent lanes over the rays of one trace (refilled as rays end).
 template <int WG, bool LDS_HIST>
 __global__ __launch_bounds__(WG) void trace_kernel(const TraceArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -106,8 +106,10 @@ __global__ __launch_bounds__(WG) void trace_kernel(const TraceArgs a) {
     lds.contrib = a.lds_surf ? s_contrib : nullptr;
     lds.senergy = a.lds_surf ? s_senergy : nullptr;
     LaneStats st;
-    for (int ray = blockIdx.x * WG + tid; ray < a.p.num_rays; ray += gridDim.x * WG)
-        trace_one_ray<WG, LDS_HIST>(a.sv, a.p, a.out, lds, ray, st, a.error_flag);
+    // persistent lanes: each wave claims rays from the trace's cursor as its
+    // lanes' rays end
+    unsigned* cursor = reinterpret_cast<unsigned*>(a.out.counters + AA_CNT_RAYS_CLAIMED);
+    trace_rays_refill<WG, LDS_HIST>(a.sv, a.p, a.out, lds, cursor, 0, a.p.num_rays, st, a.error_flag);
     flush_stats(st, a.out, a.p.num_sources);
     __syncthreads();
     if (LDS_HIST) {
@@ -379,12 +381,18 @@ int aa_stream_sync(void* stream) {
 namespace {
 
 template <int WG, bool LDS>
-hipError_t launch_trace(const aa::TraceArgs& args, int grid, size_t lds, hipStream_t stream) {
+hipError_t launch_trace(const aa::TraceArgs& args, int grid_cap, int per_cu, int cus, size_t lds, hipStream_t stream) {
     auto kern = aa::trace_kernel<WG, LDS>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
+    if (per_cu <= 0) {
+        int blocks = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, (const void*)kern, WG, lds);
+        per_cu = (e == hipSuccess && blocks > 0) ? blocks : 1;
+    }
+    const int grid = std::max(1, std::min(grid_cap, cus * per_cu));
     hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), lds, stream, args);
     return hipGetLastError();
 }
@@ -477,17 +485,17 @@ extern "C" int aa_trace(aa_scene_t scene, const aa_trace_params* p, const float*
     a.lds_surf = (Ks == 1 && stack_bytes + node_bytes + (lds_hist ? hist_bytes : 0) + surf_bytes <= lds_cap) ? 1 : 0;
     const size_t lds = stack_bytes + node_bytes + (lds_hist ? hist_bytes : 0) + (a.lds_surf ? surf_bytes : 0);
     if (lds > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes per workgroup", lds, lds_cap);
-    const int grid = (int)std::max<int64_t>(1, ((int64_t)p->num_rays + wg - 1) / wg);
+    const int grid_cap = (int)std::max<int64_t>(1, ((int64_t)p->num_rays + wg - 1) / wg);
 
     // zero the outputs with a kernel (capturable, unlike hipMemsetAsync)
     const aa::OutputExtents ext = aa::output_extents(o, p, S);
     for (int i = 0; i < 8; ++i) AA_HIP(zero_range(ext.ptr[i], ext.bytes[i], stream));
 
     hipError_t e = hipSuccess;
-#define AA_LAUNCH(W)                                                           \
-    case W:                                                                    \
-        e = lds_hist ? launch_trace<W, true>(a, grid, lds, stream)             \
-                     : launch_trace<W, false>(a, grid, lds, stream);           \
+#define AA_LAUNCH(W)                                                                   \
+    case W:                                                                            \
+        e = lds_hist ? launch_trace<W, true>(a, grid_cap, p->workgroups_per_cu, props.cu, lds, stream) \
+                     : launch_trace<W, false>(a, grid_cap, p->workgroups_per_cu, props.cu, lds, stream); \
         break;
     switch (wg) {
         AA_LAUNCH(64)

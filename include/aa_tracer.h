@@ -53,7 +53,8 @@ enum {
     AA_CNT_TRIS_SHADOW = 5,
     AA_CNT_CONTRIBUTIONS = 6,
     AA_CNT_ESCAPED = 7,
-    AA_CNT_COUNT = 8
+    AA_CNT_RAYS_CLAIMED = 8,  /* work cursor of the persistent lanes (>= num_rays) */
+    AA_CNT_COUNT = 9
 };
 
 typedef struct aa_scene_s *aa_scene_t;
@@ -123,6 +124,7 @@ typedef struct {
     int32_t block_size;          /* 0: tuned default; else 64/128/256/512 */
     int32_t lds_histogram;       /* -1: tuned default; 0: global atomics; 1: LDS privatised */
     int32_t lds_nodes;           /* -1: tuned default; else top 4-wide nodes staged in LDS */
+    int32_t workgroups_per_cu;   /* persistent grid = CUs x this (0: occupancy limit) */
     int32_t log_rays;            /* hit-log extent (0: off) */
     int32_t log_bounces;
 } aa_trace_params;

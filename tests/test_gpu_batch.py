@@ -1,0 +1,102 @@
+"""GPU: the persistent batch against single traces of the same requests.
+
+A trace must give the same result alone (``aa_trace``) and in a mixed batch:
+per-surface counts, active rays, bounce counts and query counts exactly;
+histograms to rel L1 <= 1e-5 (the order of float atomic sums differs).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from acoustic_agent_amd import hip_backend, scenes
+from acoustic_agent_amd.batch import BatchEngine, BatchTuning, TraceRequest, trace_sequential
+
+from helpers import rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(w):
+    a = w.scene.kernel_arrays()
+    return hip_backend.DeviceScene(w.mesh, np.asarray(a["reflection"]), np.asarray(a["scattering"]), device=0)
+
+
+@pytest.fixture(scope="module")
+def requests():
+    box = scenes.shoebox_workload(0)
+    plan = scenes.l_plan_workload(0, target_triangles=20000)
+    small = scenes.random_scene(5, 3000)
+    s_box, s_plan, s_small = _scene(box), _scene(plan), _scene(small)
+    rng = np.random.default_rng(3)
+    reqs = []
+    for i in range(6):
+        for w, s in ((box, s_box), (plan, s_plan), (small, s_small)):
+            ks = 1 if i % 3 else 3
+            srcs = np.stack([w.source + rng.uniform(-0.3, 0.3, 3) * (1, 1, 0.2) for _ in range(ks)])
+            reqs.append(TraceRequest(scene=s, listener=w.listener + rng.uniform(-0.05, 0.05, 3), sources=srcs,
+                                     num_rays=int(rng.integers(500, 6000)), num_bounces=int(rng.integers(20, 120)),
+                                     duration_s=float(rng.choice([0.5, 1.0, 2.0])), seed=int(rng.integers(0, 1 << 40)),
+                                     tag=len(reqs)))
+    return reqs
+
+
+def _check_same(a, b):
+    assert rel_l1(a["echogram"], b["echogram"]) <= 1e-5
+    np.testing.assert_array_equal(a["hit_counts"], b["hit_counts"])
+    np.testing.assert_array_equal(a["contrib_counts"], b["contrib_counts"])
+    np.testing.assert_array_equal(a["active_count"], b["active_count"])
+    np.testing.assert_array_equal(a["actual_bounces"], b["actual_bounces"])
+    for key in ("bounce_queries", "shadow_queries", "contributions"):
+        assert a["counters"][key] == b["counters"][key], key
+
+
+def test_batch_equals_single_traces(requests):
+    single, _ = trace_sequential(requests)
+    engine = BatchEngine(0, max_traces=64)
+    got = {r["tag"]: r for r in engine.submit(requests).wait()}
+    for s in single:
+        _check_same(got[s["tag"]], s)
+    engine.close()
+
+
+@pytest.mark.parametrize("tuning", [
+    BatchTuning(block_size=64, packet_rays=256, lds_nodes=0),
+    BatchTuning(block_size=128, packet_rays=4096, lds_nodes=5),
+    BatchTuning(block_size=512, packet_rays=1024, lds_nodes=85, workgroups_per_cu=1),
+])
+def test_batch_tunings_agree(requests, tuning):
+    engine = BatchEngine(0, max_traces=64)
+    base = {r["tag"]: r for r in engine.submit(requests).wait()}
+    other = {r["tag"]: r for r in engine.submit(requests, tuning).wait()}
+    for tag, r in base.items():
+        _check_same(other[tag], r)
+    engine.close()
+
+
+def test_graph_replays_follow_new_contents(requests):
+    import torch
+
+    engine = BatchEngine(0, max_traces=64)
+    tuning = BatchTuning(use_graph=True)
+    single = {r["tag"]: r for r in trace_sequential(requests)[0]}
+    subsets = [requests, requests[:7], requests[5:], requests]
+    for rep, subset in enumerate(subsets):
+        res = engine.submit(subset, tuning).wait()
+        assert len(res) == len(subset)
+        for r in res:
+            _check_same(r, single[r["tag"]])
+        # eager work between replays
+        x = torch.randn(1 << 16, device="cuda:0")
+        torch.cuda.synchronize()
+        assert torch.isfinite(x.sum()).item()
+    engine.close()
+
+
+def test_batch_rejects_oversize_and_empty(requests):
+    engine = BatchEngine(0, max_traces=4)
+    with pytest.raises(ValueError):
+        engine.submit([])
+    with pytest.raises(ValueError):
+        engine.submit(requests[:5])
+    engine.close()
