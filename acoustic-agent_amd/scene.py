@@ -394,22 +394,32 @@ def tessellate(arrays: Mapping[str, Any], max_edge: float | None = None) -> Tria
     )
 
 
+def _refined_count(longest: np.ndarray, max_edge: float) -> int:
+    levels = np.where(longest <= max_edge, 0, np.ceil(np.log2(np.maximum(longest, 1e-300) / max_edge)))
+    return int(np.sum(4.0 ** levels))
+
+
 def tessellate_to_count(arrays: Mapping[str, Any], target: int) -> tuple[TriangleMesh, float | None]:
     """Tessellate with the refinement edge whose triangle count is nearest ``target``."""
     coarse = tessellate(arrays, None)
     if coarse.num_triangles >= target:
         return coarse, None
-    lo, hi = 0.01, 50.0
-    best = (coarse, None)
-    for _ in range(30):
+    v = coarse.vertices.reshape(-1, 3, 3)
+    longest = np.stack([np.linalg.norm(v[:, 1] - v[:, 0], axis=1), np.linalg.norm(v[:, 2] - v[:, 1], axis=1),
+                        np.linalg.norm(v[:, 0] - v[:, 2], axis=1)], axis=1).max(axis=1)
+    lo, hi = 1e-3, float(longest.max()) * 1.01
+    best_edge, best_err = None, abs(coarse.num_triangles - target)
+    for _ in range(60):
         mid = math.sqrt(lo * hi)
-        mesh = tessellate(arrays, mid)
-        if abs(mesh.num_triangles - target) < abs(best[0].num_triangles - target):
-            best = (mesh, mid)
-        if mesh.num_triangles > target:
+        n = _refined_count(longest, mid)
+        if abs(n - target) < best_err:
+            best_edge, best_err = mid, abs(n - target)
+        if n > target:
             lo = mid
         else:
             hi = mid
-        if hi / lo < 1.02:
+        if hi / lo < 1.0005:
             break
-    return best
+    if best_edge is None:
+        return coarse, None
+    return tessellate(arrays, best_edge), best_edge

@@ -68,7 +68,7 @@ def _rect_corners(cx: float, cy: float, w: float, d: float, angle: float) -> np.
 
 
 def l_plan(seed: int = 0, num_boxes: int = 60, height: float = 2.8, door_height: float = 2.1,
-           source=L_SOURCE, listener=L_LISTENER, max_tries: int = 20000) -> RoomScene:
+           source=L_SOURCE, listener=L_LISTENER, max_tries: int = 20000, keep_out=()) -> RoomScene:
     rng = np.random.default_rng(seed)
     walls: list[WallSpec] = []
     for i in range(len(L_OUTLINE)):
@@ -86,7 +86,7 @@ def l_plan(seed: int = 0, num_boxes: int = 60, height: float = 2.8, door_height:
         walls.append(WallSpec(pt(lo), pt(hi), z_min=door_height, z_max=height, name=f"interior_{di}_lintel",
                               material=m))
     outline = np.asarray(L_OUTLINE)
-    keep_out = [np.asarray(source[:2]), np.asarray(listener[:2])]
+    keep_out = [np.asarray(source[:2]), np.asarray(listener[:2])] + [np.asarray(p[:2]) for p in keep_out]
     door_zones = []
     for axis, fixed, lo, hi in L_DOORS:
         mid = 0.5 * (lo + hi)
@@ -148,3 +148,70 @@ def random_scene(seed: int, target_triangles: int) -> Workload:
     scene = l_plan(seed, num_boxes=num_boxes)
     mesh, edge = tessellate_to_count(scene.kernel_arrays(), target_triangles)
     return Workload(scene, mesh, np.asarray(L_SOURCE), np.asarray(L_LISTENER), seed, edge)
+
+
+def circular_array(center, count: int = 4, radius: float = 0.05, orientation_deg: float = 0.0) -> np.ndarray:
+    """Channel positions of a circular microphone array (mic.py:38-49, 65-77)."""
+    c = np.asarray(center, dtype=np.float64)
+    start = math.radians(orientation_deg)
+    out = []
+    for i in range(count):
+        a = start + 2.0 * math.pi * i / count
+        out.append(c + np.asarray([round(math.cos(a) * radius, 6), round(math.sin(a) * radius, 6), 0.0]))
+    return np.asarray(out)
+
+
+def catmull_rom(points: np.ndarray, samples: int) -> np.ndarray:
+    """Uniform Catmull-Rom spline through ``points`` (end points repeated)."""
+    p = np.asarray(points, dtype=np.float64)
+    p = np.concatenate([p[:1], p, p[-1:]])
+    segs = p.shape[0] - 3
+    out = []
+    for k in range(samples):
+        u = k / max(samples - 1, 1) * segs
+        i = min(int(u), segs - 1)
+        t = u - i
+        p0, p1, p2, p3 = p[i], p[i + 1], p[i + 2], p[i + 3]
+        t2, t3 = t * t, t * t * t
+        out.append(0.5 * ((2 * p1) + (-p0 + p2) * t + (2 * p0 - 5 * p1 + 4 * p2 - p3) * t2 + (-p0 + 3 * p1 - 3 * p2 + p3) * t3))
+    return np.asarray(out)
+
+
+def moving_source_path(seed: int, keyframes: int = 64, fps: float = 32.0, start=L_SOURCE,
+                       bounds=(0.6, 0.6, 6.4, 4.4)) -> np.ndarray:
+    """Seeded source path: a Catmull-Rom spline through random control points
+    inside ``bounds`` (room 1 of the L plan), resampled to ``keyframes``
+    positions ``1/fps`` apart at a constant speed drawn from U[0.5, 1.5] m/s."""
+    rng = np.random.default_rng(seed)
+    speed = float(rng.uniform(0.5, 1.5))
+    length = speed * (keyframes - 1) / fps
+    pts = [np.asarray(start, dtype=np.float64)]
+    while True:
+        nxt = pts[-1] + np.concatenate([rng.normal(0.0, 1.0, 2), [0.0]])
+        nxt[0] = min(max(nxt[0], bounds[0]), bounds[2])
+        nxt[1] = min(max(nxt[1], bounds[1]), bounds[3])
+        pts.append(nxt)
+        dense = catmull_rom(np.asarray(pts), 50 * len(pts))
+        arc = np.concatenate([[0.0], np.cumsum(np.linalg.norm(np.diff(dense, axis=0), axis=1))])
+        if arc[-1] >= length or len(pts) > 64:
+            break
+    keep = np.concatenate([[True], np.diff(arc) > 1e-12])  # np.interp needs strictly increasing abscissae
+    arc, dense = arc[keep], dense[keep]
+    target = np.linspace(0.0, min(length, arc[-1]), keyframes)
+    out = np.stack([np.interp(target, arc, dense[:, a]) for a in range(3)], axis=1)
+    out[:, 2] = start[2]
+    return out
+
+
+def mic_array_workload(seed: int, target_triangles: int, num_mics: int = 4, radius: float = 0.05) -> tuple[Workload, np.ndarray]:
+    """A scene of the batched mix with its microphone-ring channel positions."""
+    w = random_scene(seed, target_triangles)
+    return w, circular_array(w.listener, num_mics, radius)
+
+
+def keyframe_workload(seed: int, target_triangles: int = 20000, keyframes: int = 64) -> tuple[Workload, np.ndarray]:
+    """A furnished L plan whose furniture keeps clear of a moving source path."""
+    path = moving_source_path(seed, keyframes)
+    scene = l_plan(seed, keep_out=[tuple(p) for p in path[:: max(1, keyframes // 16)]] + [tuple(path[-1])])
+    mesh, edge = tessellate_to_count(scene.kernel_arrays(), target_triangles)
+    return Workload(scene, mesh, path[0], np.asarray(L_LISTENER), seed, edge), path

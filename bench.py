@@ -62,8 +62,106 @@ def band_errors(a: np.ndarray, b: np.ndarray, bin_dur: float) -> dict[str, float
     return out
 
 
+def _batch_vs_plain(requests, label: str, steps: int, warmup: int, plain_sample: int, use_graph: bool,
+                    tune: bool) -> dict:
+    """Time the persistent batch (autotuned) against plain one-trace-per-launch."""
+    import torch
+
+    from acoustic_agent_amd.autotune import Autotuner
+    from acoustic_agent_amd.batch import BatchEngine, BatchTuning, trace_sequential
+
+    engine = BatchEngine(0, max_traces=len(requests))
+    t0 = time.perf_counter()
+    tuner = Autotuner(0)
+    tuning = tuner.tune_batch(requests, sample=min(64, len(requests)), reps=1) if tune else BatchTuning()
+    tune_s = time.perf_counter() - t0
+    tuning.use_graph = use_graph
+    for _ in range(warmup):
+        engine.submit(requests, tuning).wait()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        res = engine.submit(requests, tuning).wait()
+        times.append(time.perf_counter() - t)
+    rirs = sum(r.num_sources for r in requests)
+    rays = sum(r.num_rays for r in requests)
+    queries = sum(x["counters"]["bounce_queries"] + x["counters"]["shadow_queries"] for x in res)
+    med = float(np.median(times))
+    rng = np.random.default_rng(0)
+    pick = sorted(rng.choice(len(requests), size=min(plain_sample, len(requests)), replace=False))
+    sample = [requests[int(i)] for i in pick]
+    _, plain_s = trace_sequential(sample)
+    plain_rirs_per_s = sum(r.num_sources for r in sample) / plain_s
+    engine.close()
+    return {
+        "workload": label,
+        "traces": len(requests),
+        "rirs": rirs,
+        "batch_s_per_step": round(med, 4),
+        "rirs_per_s": round(rirs / med, 2),
+        "primary_rays_per_s": round(rays / med, 1),
+        "segment_queries_per_s": round(queries / med, 1),
+        "tuning": tuning.__dict__,
+        "autotune_s": round(tune_s, 2),
+        "plain_sample_traces": len(sample),
+        "plain_rirs_per_s": round(plain_rirs_per_s, 3),
+        "speedup_vs_plain": round((rirs / med) / plain_rirs_per_s, 2),
+    }
+
+
+def run_batched(args) -> int:
+    from acoustic_agent_amd import workloads
+
+    t = time.perf_counter()
+    if args.workload == "c2":
+        reqs = workloads.mic_array_batch(num_scenes=args.scenes or 256, seed=args.seed, num_bounces=args.bounces)
+        label = f"{args.scenes or 256} scenes (12-50k triangles, log-uniform) x 4-mic ring, 20k-200k rays, " \
+                f"{args.bounces} bounces, 2.0 s @ 1 ms"
+    else:
+        reqs = workloads.keyframe_batch(num_scenes=args.scenes or 32, seed=args.seed, num_bounces=args.bounces)
+        label = f"{args.scenes or 32} furnished L plans x 64-keyframe moving source (one multi-source trace per " \
+                f"scene), 100k rays, {args.bounces} bounces, 1.0 s @ 1 ms, hipGraph replay"
+    build_s = time.perf_counter() - t
+    out = _batch_vs_plain(reqs, label, args.steps, args.warmup, args.plain_sample, args.workload == "c3",
+                          not args.no_tune)
+    out["scene_build_s"] = round(build_s, 2)
+    print(json.dumps({"metric": "RIRs/s (batched persistent vs plain one-trace-per-launch)", "unit": "RIRs/s",
+                      "value": out["rirs_per_s"], "dtype": "f32", "data": "synthetic", "config": out}), flush=True)
+    return 0
+
+
+def run_served(args) -> int:
+    from acoustic_agent_amd import workloads
+    from acoustic_agent_amd.serving import ServingLoop, serve_sequential
+
+    loop = ServingLoop(0, slots=3, max_batch_traces=256)
+    report = {}
+    for rate in (50, 100, 200, 400):
+        stream = workloads.served_stream(rate, args.serve_seconds, seed=args.seed + rate)
+        _, served = loop.run(stream)
+        stream2 = workloads.served_stream(rate, args.serve_seconds, seed=args.seed + rate)
+        _, seq = serve_sequential(stream2)
+        report[str(rate)] = {"served": served, "sequential_plain": seq}
+        print(f"rate {rate}/s: served p50/p99 {served['latency_all']} rirs/s {served['rirs_per_s']} | "
+              f"sequential p50/p99 {seq['latency_all']} rirs/s {seq['rirs_per_s']}", file=sys.stderr, flush=True)
+    loop.close()
+    print(json.dumps({"metric": "p50/p99 served latency (continuous batching vs sequential plain)",
+                      "unit": "ms", "dtype": "f32", "data": "synthetic Poisson arrivals",
+                      "config": {"arrival_seconds": args.serve_seconds, "slots": 3, "hbm_budget_gb": 64},
+                      "rates": report}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4"), default="c1",
+                    help="c1: plain single-scene trace (default bench line); c2: batched mic arrays; "
+                         "c3: batched moving-source keyframes; c4: served Poisson stream")
+    ap.add_argument("--scenes", type=int, default=0)
+    ap.add_argument("--plain-sample", type=int, default=64, help="traces timed on the plain path (c2/c3)")
+    ap.add_argument("--serve-seconds", type=float, default=1.5)
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -85,6 +183,10 @@ def main() -> int:
     if not hip_backend.hip_available(0):
         print("no HIP device", file=sys.stderr)
         return 1
+    if args.workload in ("c2", "c3"):
+        return run_batched(args)
+    if args.workload == "c4":
+        return run_served(args)
     tuning = json.loads(args.tuning) if args.tuning else None
     N, B, bin_dur, duration = int(args.rays), int(args.bounces), 1e-3, 2.0
     K = int(round(duration / bin_dur))

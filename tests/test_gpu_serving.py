@@ -1,0 +1,71 @@
+"""GPU: the serving loop and the autotuner.
+
+Each served result must be the same alone (one plain launch per trace) and in
+the continuous-batching mix: counts exactly, histograms to rel L1 <= 1e-5.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from acoustic_agent_amd import workloads
+from acoustic_agent_amd.autotune import BATCH_SPACE, Autotuner
+from acoustic_agent_amd.batch import trace_sequential
+from acoustic_agent_amd.serving import ServingLoop, request_bytes, serve_sequential
+
+from helpers import rel_l1
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def stream():
+    return workloads.served_stream(rate_hz=150.0, duration_s=0.25, seed=11, num_scenes=4, max_rays=20_000)
+
+
+def test_served_results_equal_alone(stream):
+    loop = ServingLoop(0, slots=2, max_batch_traces=32)
+    done, summary = loop.run(stream, realtime=True)
+    loop.close()
+    assert len(done) == len(stream)
+    assert summary["latency_all"]["count"] == len(stream)
+    assert summary["latency_all"]["p99_ms"] >= summary["latency_all"]["p50_ms"] > 0
+    for comp in done:
+        alone, _ = trace_sequential(comp.request.traces)
+        assert len(alone) == len(comp.results)
+        for a, m in zip(alone, comp.results):
+            assert rel_l1(m["echogram"], a["echogram"]) <= 1e-5
+            np.testing.assert_array_equal(m["hit_counts"], a["hit_counts"])
+            np.testing.assert_array_equal(m["active_count"], a["active_count"])
+            assert m["counters"]["bounce_queries"] == a["counters"]["bounce_queries"]
+        assert comp.rt60 is not None and all(v >= 0 for v in comp.rt60.values())
+
+
+def test_hbm_budget_limits_admission(stream):
+    small = max(request_bytes(r) for r in stream)
+    loop = ServingLoop(0, slots=2, max_batch_traces=32, hbm_budget_bytes=small)
+    done, _ = loop.run(stream[:6], realtime=False)
+    assert len(done) == 6
+    loop.close()
+    loop = ServingLoop(0, slots=1, max_batch_traces=32, hbm_budget_bytes=small - 1)
+    with pytest.raises(ValueError):
+        loop.run(stream[:6], realtime=False)
+    loop.close()
+
+
+def test_sequential_baseline_reports_latency(stream):
+    done, summary = serve_sequential(stream[:4], realtime=False)
+    assert len(done) == 4 and summary["traces"] == sum(len(r.traces) for r in stream[:4])
+
+
+def test_autotuner_picks_from_space_and_caches(stream, tmp_path):
+    reqs = [t for r in stream[:6] for t in r.traces]
+    path = tmp_path / "tune.json"
+    tuner = Autotuner(0, cache_path=path)
+    space = {"block_size": (128, 256), "packet_rays": (1024, 4096), "lds_nodes": (0, 21), "workgroups_per_cu": (0,)}
+    best = tuner.tune_batch(reqs, space=space, sample=8, reps=1)
+    assert best.block_size in space["block_size"] and best.packet_rays in space["packet_rays"]
+    assert len(tuner.log) == 8
+    again = Autotuner(0, cache_path=path).tune_batch(reqs, space=space, sample=8, reps=1)
+    assert again.key() == best.key()
+    assert set(BATCH_SPACE) == set(space)
