@@ -80,6 +80,7 @@ class TraceParams(ctypes.Structure):
         ("lds_histogram", ctypes.c_int32),
         ("lds_nodes", ctypes.c_int32),
         ("workgroups_per_cu", ctypes.c_int32),
+        ("sources_per_lane", ctypes.c_int32),
         ("log_rays", ctypes.c_int32),
         ("log_bounces", ctypes.c_int32),
     ]
@@ -136,6 +137,7 @@ _SIGNATURES = {
         [_I32, ctypes.c_char_p, _I32, ctypes.c_char_p, _I32, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_I32)],
     ),
     "aa_build_bvh": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
+    "aa_build_bvh_sah": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     "aa_scene_create": (_I32, [_I32, _P, _P, _P, _I32, _P, _P, _I32, _I32, ctypes.POINTER(_P)]),
     "aa_scene_info": (_I32, [_P, ctypes.POINTER(SceneInfo)]),
     "aa_scene_release": (_I32, [_P]),

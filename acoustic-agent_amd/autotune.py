@@ -24,7 +24,7 @@ from .batch import BatchEngine, BatchTuning, TraceRequest
 
 BATCH_SPACE = {
     "block_size": (128, 256),
-    "packet_rays": (1024, 4096, 16384),
+    "packet_rays": (0, 1024, 4096),
     "lds_nodes": (0, 21, 85),
     "workgroups_per_cu": (0,),
 }

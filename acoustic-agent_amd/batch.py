@@ -85,7 +85,7 @@ class TraceRequest:
 @dataclass
 class BatchTuning:
     block_size: int = 256
-    packet_rays: int = 2048
+    packet_rays: int = 0  # 0: about two work items per resident wave
     lds_nodes: int = 21
     workgroups_per_cu: int = 0
     use_graph: bool = False

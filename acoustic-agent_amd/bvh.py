@@ -18,7 +18,8 @@ def _ptr(a: np.ndarray) -> ctypes.c_void_p:
     return ctypes.c_void_p(a.ctypes.data)
 
 
-def build_bvh(bounds_min: np.ndarray, bounds_max: np.ndarray, *, leaf_size: int = 4) -> dict[str, Any]:
+def build_bvh(bounds_min: np.ndarray, bounds_max: np.ndarray, *, leaf_size: int = 4, sah: bool = False) -> dict[str, Any]:
+    """``sah=True`` builds the device tree (binned SAH) in the same layout."""
     lower = np.ascontiguousarray(bounds_min, dtype=np.float64)
     upper = np.ascontiguousarray(bounds_max, dtype=np.float64)
     if lower.shape != upper.shape or lower.ndim != 2 or lower.shape[1] != 3:
@@ -35,7 +36,7 @@ def build_bvh(bounds_min: np.ndarray, bounds_max: np.ndarray, *, leaf_size: int 
     prims = np.zeros(n, dtype=np.int64)
     stats = np.zeros(3, dtype=np.int64)
     lib = _lib.load()
-    _lib.check(lib.aa_build_bvh(_ptr(lower), _ptr(upper), n, max(1, int(leaf_size)), _ptr(node_min), _ptr(node_max),
+    _lib.check((lib.aa_build_bvh_sah if sah else lib.aa_build_bvh)(_ptr(lower), _ptr(upper), n, max(1, int(leaf_size)), _ptr(node_min), _ptr(node_max),
                                 _ptr(start), _ptr(count), _ptr(escape), _ptr(prims), _ptr(stats)))
     nodes = int(stats[0])
     return {

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(WG) void batch_kernel(BatchHeader* hdr, const Batch
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         const int begin = (item - bt.first_packet) * packet_rays;
-        const int count = min(packet_rays, bt.p.num_rays - begin);
+        const int count = min(packet_rays, bt.p.num_rays * bt.p.groups - begin);
         Lds lds;
         lds.stack = s_stack;
         lds.nodes = my_nodes;
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(WG) void batch_kernel(BatchHeader* hdr, const Batch
         lds.senergy = lds_surf ? my_energy : nullptr;
         LaneStats st;
         trace_rays_refill<WG, false>(sv, bt.p, bt.out, lds, &s_cursor[wave], begin, count, st, bt.error_flag);
-        flush_stats(st, bt.out, bt.p.num_sources);
+        flush_stats(st, bt.p, bt.out);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (lds_surf) {
@@ -242,8 +242,7 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
     if (count < 1 || count > b->max_traces) return fail(AA_ERR_INVALID, "count must be in [1, %d]", b->max_traces);
     const int wg = tuning->block_size > 0 ? tuning->block_size : 256;
     if (wg != 64 && wg != 128 && wg != 256 && wg != 512) return fail(AA_ERR_INVALID, "block_size must be 64..512");
-    const int packet_rays = tuning->packet_rays > 0 ? tuning->packet_rays : 1024;
-    if (packet_rays < 1 || packet_rays > (1 << 24)) return fail(AA_ERR_INVALID, "packet_rays out of range");
+    if (tuning->packet_rays > (1 << 24)) return fail(AA_ERR_INVALID, "packet_rays out of range");
     for (int32_t i = 0; i < count; ++i) {
         const aa_batch_trace& t = traces[i];
         int rc = aa::validate_trace(t.scene, &t.params, t.d_sources, t.d_directions, t.d_diffuse_bank,
@@ -274,6 +273,31 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
     const size_t lds = base + (size_t)waves * surf_cap * 12;
     if (lds > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", lds, lds_cap);
 
+    int per_cu = tuning->workgroups_per_cu;
+    if (per_cu <= 0) {
+        switch (wg) {
+            case 64: per_cu = occupancy_blocks<64>(lds); break;
+            case 128: per_cu = occupancy_blocks<128>(lds); break;
+            case 256: per_cu = occupancy_blocks<256>(lds); break;
+            default: per_cu = occupancy_blocks<512>(lds); break;
+        }
+    }
+    // packet size: the tuning's, or (0) about two work items per resident
+    // wave so that small batches still spread over the whole GPU
+    int64_t tasks = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const aa_batch_trace& t = traces[i];
+        const int g = t.params.sources_per_lane > 0 ? std::min(t.params.sources_per_lane, t.params.num_sources)
+                                                    : std::min(8, t.params.num_sources);
+        tasks += (int64_t)t.params.num_rays * ((t.params.num_sources + g - 1) / g);
+    }
+    int packet_rays = tuning->packet_rays;
+    if (packet_rays <= 0) {
+        const int64_t resident = (int64_t)props.cu * per_cu * waves;
+        const int64_t want = (tasks + 2 * resident - 1) / (2 * resident);
+        packet_rays = (int)std::min<int64_t>(16384, std::max<int64_t>(64, (want + 63) / 64 * 64));
+    }
+
     // the pinned table may still be read by the previous submission's copy
     if (b->pending) AA_HIP(hipEventSynchronize(b->copied));
     const size_t hdr_bytes = (sizeof(aa::BatchHeader) + 255) & ~size_t(255);
@@ -289,7 +313,9 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
                                   t.d_diffuse_indices);
         d.out = aa::make_outputs(&t.outputs, &t.params);
         d.error_flag = t.scene->d_error;
-        d.num_packets = (t.params.num_rays + packet_rays - 1) / packet_rays;
+        const int64_t trace_tasks = (int64_t)d.p.num_rays * d.p.groups;
+        if (trace_tasks > (int64_t)1 << 30) return fail(AA_ERR_LIMIT, "trace %d has too many lane tasks", i);
+        d.num_packets = (int)((trace_tasks + packet_rays - 1) / packet_rays);
         d.first_packet = (int)packets;
         packets += d.num_packets;
         d.lds_surf = (t.params.num_sources == 1 && t.scene->num_surfaces <= surf_cap) ? 1 : 0;
@@ -307,15 +333,6 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
 
     aa::BatchHeader* d_hdr = static_cast<aa::BatchHeader*>(b->d_table);
     const aa::BatchTrace* d_tr = reinterpret_cast<const aa::BatchTrace*>(static_cast<char*>(b->d_table) + hdr_bytes);
-    int per_cu = tuning->workgroups_per_cu;
-    if (per_cu <= 0) {
-        switch (wg) {
-            case 64: per_cu = occupancy_blocks<64>(lds); break;
-            case 128: per_cu = occupancy_blocks<128>(lds); break;
-            case 256: per_cu = occupancy_blocks<256>(lds); break;
-            default: per_cu = occupancy_blocks<512>(lds); break;
-        }
-    }
     const int wave_items = (int)std::min<int64_t>(packets, (int64_t)1 << 30);
     const int grid = std::max(1, std::min(props.cu * per_cu, (wave_items + waves - 1) / waves));
     const int persistent_grid = std::max(1, props.cu * per_cu);

@@ -90,6 +90,145 @@ double half_area(const double* lo, const double* hi) {
     return ex * ey + ey * ez + ez * ex;
 }
 
+// Binned surface-area-heuristic builder producing the same preorder /
+// escape layout as the canonical builder.
+struct SahBuilder {
+    static constexpr int kBins = 16;
+    static constexpr double kTraversalCost = 1.2;
+    static constexpr double kTriangleCost = 1.0;
+    const double* lower;
+    const double* upper;
+    std::vector<double> centroid;
+    int64_t max_leaf;
+    CanonicalBVH* out;
+
+    void bounds_of(const std::vector<int64_t>& idx, size_t b, size_t e, double* lo, double* hi) const {
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = INFINITY;
+            hi[a] = -INFINITY;
+        }
+        for (size_t k = b; k < e; ++k)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], lower[idx[k] * 3 + a]);
+                hi[a] = std::max(hi[a], upper[idx[k] * 3 + a]);
+            }
+    }
+
+    void make_leaf(std::vector<int64_t>& idx, size_t begin, size_t end, int64_t node) {
+        std::vector<int64_t> ordered(idx.begin() + (long)begin, idx.begin() + (long)end);
+        std::sort(ordered.begin(), ordered.end());
+        out->start[node] = (int64_t)out->prims.size();
+        out->count[node] = (int64_t)ordered.size();
+        out->prims.insert(out->prims.end(), ordered.begin(), ordered.end());
+        out->leaf_count += 1;
+    }
+
+    void append(std::vector<int64_t>& idx, size_t begin, size_t end, int64_t depth) {
+        const int64_t node = out->node_count();
+        double lo[3], hi[3];
+        bounds_of(idx, begin, end, lo, hi);
+        for (int a = 0; a < 3; ++a) {
+            out->lo.push_back(lo[a]);
+            out->hi.push_back(hi[a]);
+        }
+        out->start.push_back(-1);
+        out->count.push_back(0);
+        out->escape.push_back(-1);
+        out->max_depth = std::max(out->max_depth, depth);
+        const int64_t n = (int64_t)(end - begin);
+        if (n <= 1) {
+            make_leaf(idx, begin, end, node);
+            out->escape[node] = out->node_count();
+            return;
+        }
+        double cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (size_t k = begin; k < end; ++k)
+            for (int a = 0; a < 3; ++a) {
+                cmin[a] = std::min(cmin[a], centroid[idx[k] * 3 + a]);
+                cmax[a] = std::max(cmax[a], centroid[idx[k] * 3 + a]);
+            }
+        const double parent_area = std::max(half_area(lo, hi), 1e-300);
+        double best_cost = INFINITY;
+        int best_axis = -1, best_bin = -1;
+        for (int a = 0; a < 3; ++a) {
+            const double extent = cmax[a] - cmin[a];
+            if (!(extent > 1e-12)) continue;
+            int64_t cnt[kBins] = {0};
+            double blo[kBins][3], bhi[kBins][3];
+            for (int b = 0; b < kBins; ++b)
+                for (int c = 0; c < 3; ++c) {
+                    blo[b][c] = INFINITY;
+                    bhi[b][c] = -INFINITY;
+                }
+            for (size_t k = begin; k < end; ++k) {
+                const int64_t p = idx[k];
+                int b = (int)((centroid[p * 3 + a] - cmin[a]) / extent * kBins);
+                b = std::min(std::max(b, 0), kBins - 1);
+                cnt[b] += 1;
+                for (int c = 0; c < 3; ++c) {
+                    blo[b][c] = std::min(blo[b][c], lower[p * 3 + c]);
+                    bhi[b][c] = std::max(bhi[b][c], upper[p * 3 + c]);
+                }
+            }
+            double right_area[kBins];
+            int64_t right_cnt[kBins];
+            double rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            int64_t rc = 0;
+            for (int b = kBins - 1; b > 0; --b) {
+                rc += cnt[b];
+                for (int c = 0; c < 3; ++c) {
+                    rlo[c] = std::min(rlo[c], blo[b][c]);
+                    rhi[c] = std::max(rhi[c], bhi[b][c]);
+                }
+                right_cnt[b] = rc;
+                right_area[b] = rc ? half_area(rlo, rhi) : 0.0;
+            }
+            double llo[3] = {INFINITY, INFINITY, INFINITY}, lhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            int64_t lc = 0;
+            for (int b = 0; b < kBins - 1; ++b) {
+                lc += cnt[b];
+                for (int c = 0; c < 3; ++c) {
+                    llo[c] = std::min(llo[c], blo[b][c]);
+                    lhi[c] = std::max(lhi[c], bhi[b][c]);
+                }
+                if (lc == 0 || right_cnt[b + 1] == 0) continue;
+                const double cost = kTraversalCost + kTriangleCost * (half_area(llo, lhi) * (double)lc +
+                                                                      right_area[b + 1] * (double)right_cnt[b + 1]) /
+                                                         parent_area;
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_axis = a;
+                    best_bin = b;
+                }
+            }
+        }
+        const double leaf_cost = kTriangleCost * (double)n;
+        if (n <= max_leaf && !(best_cost < leaf_cost)) {
+            make_leaf(idx, begin, end, node);
+            out->escape[node] = out->node_count();
+            return;
+        }
+        size_t mid;
+        if (best_axis >= 0) {
+            const double extent = cmax[best_axis] - cmin[best_axis];
+            const double* cen = centroid.data();
+            const int axis = best_axis, split = best_bin;
+            auto it = std::stable_partition(idx.begin() + (long)begin, idx.begin() + (long)end, [&](int64_t p) {
+                int b = (int)((cen[p * 3 + axis] - cmin[axis]) / extent * kBins);
+                b = std::min(std::max(b, 0), kBins - 1);
+                return b <= split;
+            });
+            mid = (size_t)(it - idx.begin());
+        } else {
+            mid = begin + (size_t)(n / 2);  // coincident centroids: split the list
+        }
+        if (mid == begin || mid == end) mid = begin + (size_t)(n / 2);
+        append(idx, begin, mid, depth + 1);
+        append(idx, mid, end, depth + 1);
+        out->escape[node] = out->node_count();
+    }
+};
+
 float round_down(double x) {
     float f = (float)x;
     if ((double)f > x) f = std::nextafter(f, -INFINITY);
@@ -121,6 +260,26 @@ std::string build_canonical_bvh(const double* lower, const double* upper, int64_
     b.out = &out;
     out.lo.reserve((size_t)(6 * n));
     out.hi.reserve((size_t)(6 * n));
+    std::vector<int64_t> idx((size_t)n);
+    std::iota(idx.begin(), idx.end(), 0);
+    b.append(idx, 0, (size_t)n, 1);
+    return std::string();
+}
+
+std::string build_sah_bvh(const double* lower, const double* upper, int64_t n, int64_t leaf_size, CanonicalBVH& out) {
+    out = CanonicalBVH();
+    if (n < 1) return "BVH requires at least one primitive";
+    for (int64_t k = 0; k < n * 3; ++k) {
+        if (!std::isfinite(lower[k]) || !std::isfinite(upper[k])) return "BVH bounds must be finite";
+        if (upper[k] < lower[k]) return "BVH maximum bounds must not be below minimum bounds";
+    }
+    SahBuilder b;
+    b.lower = lower;
+    b.upper = upper;
+    b.max_leaf = std::max<int64_t>(1, leaf_size);
+    b.centroid.resize((size_t)(n * 3));
+    for (int64_t k = 0; k < n * 3; ++k) b.centroid[(size_t)k] = 0.5 * (lower[k] + upper[k]);
+    b.out = &out;
     std::vector<int64_t> idx((size_t)n);
     std::iota(idx.begin(), idx.end(), 0);
     b.append(idx, 0, (size_t)n, 1);

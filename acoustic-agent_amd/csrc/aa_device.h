@@ -53,6 +53,8 @@ struct RayParams {
     const int* uidx;
     const float4* sources;  // x, y, z, direct delay
     int num_rays, num_bounces, num_bins, num_samples, num_sources, rng_mode, amb;
+    int src_group;  // sources handled by one lane (one task = one ray x one group)
+    int groups;     // ceil(num_sources / src_group); tasks = num_rays * groups, group-major
     unsigned seed_lo, seed_hi;
     float lx, ly, lz, fx, fy, fz;
     float bin_dur, c, max_path_len, listener_radius, source_radius, dmin, spec_exp, dip_w, dip_p;
@@ -76,8 +78,9 @@ struct Outputs {
 // Per-lane tallies flushed once per wave.
 struct LaneStats {
     unsigned q_b = 0, q_s = 0, nb = 0, ns = 0, tb = 0, ts = 0, contrib = 0, escaped = 0;
-    int max_iter_full = 0;  // bounce iterations of rays that ended with every source alive
+    int max_iter_full = 0;  // bounce iterations of rays that ended with every source of `grp` alive
     int active_full = 0;
+    int grp = -1;           // source group of max_iter_full / active_full (multi-group traces)
 };
 
 // LDS views of one workgroup.
@@ -405,10 +408,35 @@ struct RayState {
     float d0x, d0y, d0z;  // initial direction (FOA weights)
     float A[kBands];      // accumulated energy reflection per band
     float accum;          // path length so far
-    unsigned long long mask;  // sources for which the ray is still alive
+    unsigned long long mask;        // sources for which the ray is still alive
+    unsigned long long group_mask;  // the sources of this task's group
     int ray;
+    int group;
     int bounce;
 };
+
+__device__ __forceinline__ unsigned long long group_source_mask(const RayParams& p, int group) {
+    const int first = group * p.src_group;
+    const int last = min(p.num_sources, first + p.src_group);
+    const unsigned long long upto = last >= 64 ? ~0ull : ((1ull << last) - 1ull);
+    const unsigned long long below = (1ull << first) - 1ull;
+    return upto & ~below;
+}
+
+// Per-source end-of-ray tallies of one lane, written out directly.
+__device__ __forceinline__ void flush_group_tallies(const RayParams& p, const Outputs& out, LaneStats& st) {
+    if (st.grp >= 0 && (st.max_iter_full || st.active_full)) {
+        unsigned long long m = group_source_mask(p, st.grp);
+        while (m) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1ull;
+            if (st.max_iter_full) atomicMax(&out.actual_bounces[k], st.max_iter_full);
+            if (st.active_full) atomicAdd(&out.active_count[k], st.active_full);
+        }
+    }
+    st.max_iter_full = 0;
+    st.active_full = 0;
+}
 
 __device__ __forceinline__ unsigned long long full_source_mask(int Ks) {
     return Ks >= 64 ? ~0ull : ((1ull << Ks) - 1ull);
@@ -418,7 +446,11 @@ __device__ __forceinline__ unsigned long long full_source_mask(int Ks) {
 // and the active count, per source.
 __device__ __forceinline__ void finish_ray(const RayParams& p, const Outputs& out, const RayState& s, bool alive,
                                            int iter, LaneStats& st) {
-    if (s.mask == full_source_mask(p.num_sources)) {
+    if (s.mask == s.group_mask) {
+        if (p.groups > 1 && st.grp != s.group) {
+            flush_group_tallies(p, out, st);
+            st.grp = s.group;
+        }
         st.max_iter_full = max(st.max_iter_full, iter);
         if (alive) ++st.active_full;
     } else {
@@ -433,9 +465,13 @@ __device__ __forceinline__ void finish_ray(const RayParams& p, const Outputs& ou
 }
 
 // Start `ray`; returns false if it has no bounce to trace (num_bounces == 0).
-__device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out, int ray, RayState& s,
+__device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out, int task, RayState& s,
                                           LaneStats& st) {
+    const int group = task / p.num_rays;  // group-major: a wave's claims mostly share a group
+    const int ray = task - group * p.num_rays;
     s.ray = ray;
+    s.group = group;
+    s.group_mask = group_source_mask(p, group);
     s.bounce = 0;
     s.ox = p.lx;
     s.oy = p.ly;
@@ -447,7 +483,7 @@ __device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out
 #pragma unroll
     for (int b = 0; b < kBands; ++b) s.A[b] = 1.0f;
     s.accum = 0.0f;
-    s.mask = full_source_mask(p.num_sources);
+    s.mask = s.group_mask;
     if (p.num_bounces <= 0) {
         finish_ray(p, out, s, true, 0, st);
         return false;
@@ -480,7 +516,7 @@ __device__ bool step_ray(const SceneView& sv, const RayParams& p, const Outputs&
     Closest h;
     ++st.q_b;
     closest_hit<WG>(sv, lds.nodes, stk, r, h, st.nb, st.tb, error_flag);
-    if (out.log_tri != nullptr && s.ray < out.log_rays && bounce < out.log_bounces) {
+    if (out.log_tri != nullptr && s.group == 0 && s.ray < out.log_rays && bounce < out.log_bounces) {
         out.log_tri[(size_t)s.ray * out.log_bounces + bounce] = h.tri;
         out.log_t[(size_t)s.ray * out.log_bounces + bounce] = h.t;
     }
@@ -706,7 +742,9 @@ __device__ void trace_one_ray(const SceneView& sv, const RayParams& p, const Out
 }
 
 // Wave reduction of the lane tallies, one atomic per wave and counter.
-__device__ __forceinline__ void flush_stats(LaneStats& st, const Outputs& out, int num_sources) {
+__device__ __forceinline__ void flush_stats(LaneStats& st, const RayParams& p, const Outputs& out) {
+    const int num_sources = p.num_sources;
+    if (p.groups > 1) flush_group_tallies(p, out, st);
     unsigned long long vals[8] = {st.q_b, st.q_s, st.nb, st.ns, st.tb, st.ts, st.contrib, st.escaped};
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
@@ -723,7 +761,7 @@ __device__ __forceinline__ void flush_stats(LaneStats& st, const Outputs& out, i
 #pragma unroll
         for (int c = 0; c < 8; ++c)
             if (vals[c]) atomicAdd(&out.counters[c], vals[c]);
-        for (int k = 0; k < num_sources; ++k) {
+        for (int k = 0; k < (p.groups > 1 ? 0 : num_sources); ++k) {
             if (mi) atomicMax(&out.actual_bounces[k], mi);
             if (ac) atomicAdd(&out.active_count[k], ac);
         }
@@ -763,6 +801,9 @@ inline RayParams make_ray_params(const aa_trace_params* p, const float* sources,
     r.num_bins = p->num_bins;
     r.num_samples = p->num_diffuse_samples;
     r.num_sources = p->num_sources;
+    r.src_group = p->sources_per_lane > 0 ? p->sources_per_lane : 8;
+    if (r.src_group > r.num_sources) r.src_group = r.num_sources;
+    r.groups = (r.num_sources + r.src_group - 1) / r.src_group;
     r.rng_mode = p->rng_mode;
     r.amb = p->render_ambisonics ? 1 : 0;
     r.seed_lo = (unsigned)(p->seed & 0xffffffffull);

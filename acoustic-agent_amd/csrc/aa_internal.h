@@ -28,6 +28,11 @@ struct CanonicalBVH {
 std::string build_canonical_bvh(const double* lower, const double* upper, int64_t n, int64_t leaf_size,
                                 CanonicalBVH& out);
 
+// Binned-SAH binary tree in the same layout (used for the device tree: the
+// traversal result does not depend on the tree, only its cost does).
+std::string build_sah_bvh(const double* lower, const double* upper, int64_t n, int64_t leaf_size,
+                          CanonicalBVH& out);
+
 // 4-wide BVH node as stored on the device: 8 x float4 = 128 bytes.
 //   v[0] lo_x[4]  v[1] hi_x[4]  v[2] lo_y[4]  v[3] hi_y[4]
 //   v[4] lo_z[4]  v[5] hi_z[4]  v[6] child[4] (int32 bits)  v[7] unused

@@ -109,8 +109,8 @@ __global__ __launch_bounds__(WG) void trace_kernel(const TraceArgs a) {
     // persistent lanes: each wave claims rays from the trace's cursor as its
     // lanes' rays end
     unsigned* cursor = reinterpret_cast<unsigned*>(a.out.counters + AA_CNT_RAYS_CLAIMED);
-    trace_rays_refill<WG, LDS_HIST>(a.sv, a.p, a.out, lds, cursor, 0, a.p.num_rays, st, a.error_flag);
-    flush_stats(st, a.out, a.p.num_sources);
+    trace_rays_refill<WG, LDS_HIST>(a.sv, a.p, a.out, lds, cursor, 0, a.p.num_rays * a.p.groups, st, a.error_flag);
+    flush_stats(st, a.p, a.out);
     __syncthreads();
     if (LDS_HIST) {
         for (int i = tid; i < hist_words; i += WG) {
@@ -208,13 +208,14 @@ int aa_device_info(int32_t device, char* name, int32_t name_len, char* arch, int
     return AA_OK;
 }
 
-int aa_build_bvh(const double* bounds_min, const double* bounds_max, int64_t n, int64_t leaf_size, double* node_min,
+static int build_bvh_common(bool sah, const double* bounds_min, const double* bounds_max, int64_t n, int64_t leaf_size, double* node_min,
                  double* node_max, int64_t* start, int64_t* count, int64_t* escape, int64_t* primitives,
                  int64_t* stats) {
     if (!bounds_min || !bounds_max || !node_min || !node_max || !start || !count || !escape || !primitives || !stats)
         return fail(AA_ERR_INVALID, "aa_build_bvh: NULL argument");
     aa::CanonicalBVH bvh;
-    std::string err = aa::build_canonical_bvh(bounds_min, bounds_max, n, leaf_size, bvh);
+    std::string err = sah ? aa::build_sah_bvh(bounds_min, bounds_max, n, leaf_size, bvh)
+                          : aa::build_canonical_bvh(bounds_min, bounds_max, n, leaf_size, bvh);
     if (!err.empty()) return fail(AA_ERR_INVALID, "%s", err.c_str());
     const size_t nodes = (size_t)bvh.node_count();
     std::memcpy(node_min, bvh.lo.data(), nodes * 3 * sizeof(double));
@@ -227,6 +228,20 @@ int aa_build_bvh(const double* bounds_min, const double* bounds_max, int64_t n, 
     stats[1] = bvh.leaf_count;
     stats[2] = bvh.max_depth;
     return AA_OK;
+}
+
+int aa_build_bvh(const double* bounds_min, const double* bounds_max, int64_t n, int64_t leaf_size, double* node_min,
+                 double* node_max, int64_t* start, int64_t* count, int64_t* escape, int64_t* primitives,
+                 int64_t* stats) {
+    return build_bvh_common(false, bounds_min, bounds_max, n, leaf_size, node_min, node_max, start, count, escape,
+                            primitives, stats);
+}
+
+int aa_build_bvh_sah(const double* bounds_min, const double* bounds_max, int64_t n, int64_t leaf_size,
+                     double* node_min, double* node_max, int64_t* start, int64_t* count, int64_t* escape,
+                     int64_t* primitives, int64_t* stats) {
+    return build_bvh_common(true, bounds_min, bounds_max, n, leaf_size, node_min, node_max, start, count, escape,
+                            primitives, stats);
 }
 
 int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tri_surface, const float* tri_normals,
@@ -265,8 +280,13 @@ int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tr
             hi[(size_t)k * 3 + a] = mx + (double)aa::kBvhEps;
         }
     }
+    // The reference's median-split tree (canonical; its size is reported) and
+    // the SAH tree the device walks (same hits, fewer node visits).
+    aa::CanonicalBVH canonical;
+    std::string err = aa::build_canonical_bvh(lo.data(), hi.data(), num_triangles, leaf_size, canonical);
+    if (!err.empty()) return fail(AA_ERR_INVALID, "%s", err.c_str());
     aa::CanonicalBVH bin;
-    std::string err = aa::build_canonical_bvh(lo.data(), hi.data(), num_triangles, leaf_size, bin);
+    err = aa::build_sah_bvh(lo.data(), hi.data(), num_triangles, leaf_size, bin);
     if (!err.empty()) return fail(AA_ERR_INVALID, "%s", err.c_str());
 
     aa_scene_s* s = new aa_scene_s();
@@ -274,8 +294,8 @@ int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tr
     s->num_triangles = num_triangles;
     s->num_surfaces = num_surfaces;
     s->leaf_size = leaf_size;
-    s->canonical_nodes = (int32_t)bin.node_count();
-    s->canonical_depth = (int32_t)bin.max_depth;
+    s->canonical_nodes = (int32_t)canonical.node_count();
+    s->canonical_depth = (int32_t)canonical.max_depth;
     aa::collapse_to_wide(bin, s->wide);
     if (3 * (s->wide.depth - 1) > aa::kStack) {
         free_scene(s);

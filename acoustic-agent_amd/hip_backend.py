@@ -313,6 +313,7 @@ def make_params(*, num_rays: int, num_bounces: int, num_bins: int, num_diffuse_s
     p.lds_histogram = int(t.get("lds_histogram", -1))
     p.lds_nodes = int(t.get("lds_nodes", -1))
     p.workgroups_per_cu = int(t.get("workgroups_per_cu", 0))
+    p.sources_per_lane = int(t.get("sources_per_lane", 0))
     p.log_rays = int(log_rays)
     p.log_bounces = int(log_bounces)
     return p

@@ -73,6 +73,13 @@ int aa_build_bvh(const double *bounds_min, const double *bounds_max, int64_t n, 
                  double *node_min, double *node_max, int64_t *start, int64_t *count, int64_t *escape,
                  int64_t *primitives, int64_t *stats);
 
+/* The binned-SAH tree the device traverses (no reference counterpart: the
+ * reference walks its median-split tree; the hit a query returns does not
+ * depend on the tree).  Same arguments and output layout as aa_build_bvh. */
+int aa_build_bvh_sah(const double *bounds_min, const double *bounds_max, int64_t n, int64_t leaf_size,
+                     double *node_min, double *node_max, int64_t *start, int64_t *count, int64_t *escape,
+                     int64_t *primitives, int64_t *stats);
+
 /* ---- scene (replaces _scene_kernel_arrays + the device input cache,
  *      steam_rt.py:834-924 and steam_rt_cuda.py:229-261) ----
  * Triangles carry the reference surface id they tessellate; reflection is
@@ -125,6 +132,8 @@ typedef struct {
     int32_t lds_histogram;       /* -1: tuned default; 0: global atomics; 1: LDS privatised */
     int32_t lds_nodes;           /* -1: tuned default; else top 4-wide nodes staged in LDS */
     int32_t workgroups_per_cu;   /* persistent grid = CUs x this (0: occupancy limit) */
+    int32_t sources_per_lane;    /* multi-source traces: sources sharing one lane's trajectory
+                                    (0: 8); the trace runs num_rays x ceil(Ks / this) lane tasks */
     int32_t log_rays;            /* hit-log extent (0: off) */
     int32_t log_bounces;
 } aa_trace_params;

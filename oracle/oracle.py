@@ -162,7 +162,7 @@ class OracleScene:
     """Arrays of one scene in one precision, kept alive for the C structs."""
 
     def __init__(self, arrays: Mapping[str, Any], mesh=None, precision: str = "float64", use_bvh: bool = True,
-                 leaf_size: int = 4) -> None:
+                 leaf_size: int = 4, bvh: Mapping[str, Any] | None = None) -> None:
         self.precision = precision
         real = np.float64 if precision == "float64" else np.float32
         Scene, _, _ = _STRUCTS[precision]
@@ -198,7 +198,8 @@ class OracleScene:
             v = tri_v.astype(np.float64).reshape(-1, 3, 3)
             lo, hi = v.min(axis=1) - pad, v.max(axis=1) + pad
             self.num_primitives = sc.num_triangles
-        bvh = build_bvh(lo, hi, leaf_size)
+        if bvh is None:
+            bvh = build_bvh(lo, hi, leaf_size)
         self.bvh = bvh
         sc.use_bvh = 1 if use_bvh else 0
         sc.num_nodes = int(bvh["node_count"])

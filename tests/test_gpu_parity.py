@@ -111,11 +111,12 @@ def test_launch_shapes_agree(lplan, tuning):
     assert rel_l1(other["echogram"], base["echogram"]) <= 1e-5
 
 
-def test_multi_source_equals_separate_traces(lplan):
+@pytest.mark.parametrize("sources_per_lane", [0, 2])
+def test_multi_source_equals_separate_traces(lplan, sources_per_lane):
     rng = np.random.default_rng(9)
     srcs = np.stack([np.asarray(scenes.L_SOURCE) + rng.uniform(-0.5, 0.5, 3) * (1, 1, 0.2) for _ in range(5)])
     case = Case(lplan, num_rays=2048, num_bounces=80, num_bins=1000, seed=7, rng="counter", sources=srcs)
-    multi = run_gpu(case)
+    multi = run_gpu(case, tuning={"sources_per_lane": sources_per_lane})
     for k in range(srcs.shape[0]):
         single = run_gpu(case, source=srcs[k])
         assert rel_l1(multi["echogram"][k], single["echogram"]) <= 1e-5

@@ -42,7 +42,7 @@ def test_served_results_equal_alone(stream):
 
 
 def test_hbm_budget_limits_admission(stream):
-    small = max(request_bytes(r) for r in stream)
+    small = max(request_bytes(r) for r in stream[:6])
     loop = ServingLoop(0, slots=2, max_batch_traces=32, hbm_budget_bytes=small)
     done, _ = loop.run(stream[:6], realtime=False)
     assert len(done) == 6
