@@ -72,13 +72,15 @@ __global__ void batch_zero_kernel(const BatchHeader* hdr, const BatchTrace* trac
 // Work items are claimed per wave, so that a wave whose rays run long does
 // not hold up the other waves of its workgroup; each wave has its own slot of
 // staged BVH nodes, its own surface tallies and its own ray cursor in LDS.
+// Registers are held to 3 waves per SIMD, as for the single-trace kernel.
 template <int WG>
-__global__ __launch_bounds__(WG) void batch_kernel(BatchHeader* hdr, const BatchTrace* traces, int lds_nodes_cap,
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void batch_kernel(BatchHeader* hdr, const BatchTrace* traces, int lds_nodes_cap,
                                                    int surf_cap) {
     constexpr int kWaves = WG / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned short* s_stack = reinterpret_cast<unsigned short*>(smem);
-    float4* s_nodes = reinterpret_cast<float4*>(smem + kStackBytes<WG>());
+    float* s_queue = reinterpret_cast<float*>(smem + stack_only_bytes(WG));
+    float4* s_nodes = reinterpret_cast<float4*>(smem + private_bytes(WG));
     int* s_surf = reinterpret_cast<int*>(s_nodes + (size_t)kWaves * lds_nodes_cap * 8);
     __shared__ unsigned s_cursor[kWaves];
 
@@ -129,6 +131,7 @@ __global__ __launch_bounds__(WG) void batch_kernel(BatchHeader* hdr, const Batch
         const int count = min(packet_rays, bt.p.num_rays * bt.p.groups - begin);
         Lds lds;
         lds.stack = s_stack;
+        lds.queue = s_queue;
         lds.nodes = my_nodes;
         lds.hist = nullptr;
         lds.hits = lds_surf ? my_hits : nullptr;
@@ -265,13 +268,16 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
         if (t.params.num_sources == 1) max_surf = std::max(max_surf, t.scene->num_surfaces);
     }
     const int waves = wg / 64;
-    const int lds_nodes = std::min(std::max(tuning->lds_nodes, 0), max_nodes);
     const size_t lds_cap = props.lds_per_block > 0 ? props.lds_per_block : 65536;
-    const size_t base = aa::stack_bytes(wg) + (size_t)waves * lds_nodes * sizeof(aa::WideNode);
     const size_t surf_budget = 32 * 1024;
     const int surf_cap = (max_surf > 0 && (size_t)waves * max_surf * 12 <= surf_budget) ? max_surf : 0;
-    const size_t lds = base + (size_t)waves * surf_cap * 12;
-    if (lds > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", lds, lds_cap);
+    const size_t base = aa::private_bytes(wg) + (size_t)waves * surf_cap * 12;
+    if (base > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", base, lds_cap);
+    // staged nodes only speed traversal up: take what the tuning asks for, as
+    // far as it fits next to the stacks, queues and tallies
+    const int node_room = (int)((lds_cap - base) / ((size_t)waves * sizeof(aa::WideNode)));
+    const int lds_nodes = std::min({std::max(tuning->lds_nodes, 0), max_nodes, node_room});
+    const size_t lds = base + (size_t)waves * lds_nodes * sizeof(aa::WideNode);
 
     int per_cu = tuning->workgroups_per_cu;
     if (per_cu <= 0) {

@@ -1,7 +1,9 @@
 // Device code shared by the single-trace kernel (aa_tracer.hip) and the
 // persistent batch kernel (aa_batch.hip): BVH traversal, triangle test,
-// counter RNG and the per-ray bounce loop of the reference tracer
-// (acoustic_agent/steam_rt.py:4724-4849).
+// counter RNG and the bounce loop of the reference tracer
+// (acoustic_agent/steam_rt.py:4724-4849), run as a per-lane state machine
+// (one traversal step per iteration) with a per-wave queue of shadow
+// connections.
 //
 // Floating-point contract: built with -ffp-contract=off and IEEE division /
 // sqrt, so every trajectory operation rounds like the binary32 CPU
@@ -24,13 +26,20 @@ constexpr float kHitOffset = 1e-2f; // steam_rt_cuda.py:20
 constexpr float kBvhEps = 1e-5f;    // steam_rt_cuda.py:21
 constexpr float kBig = 1e30f;
 
-// LDS bytes of the per-lane stacks, rounded up to 16 so the float4 node
-// staging that follows stays aligned.
-template <int WG>
-__host__ __device__ constexpr size_t kStackBytes() {
-    return ((size_t)kStack * WG * sizeof(unsigned short) + 15) & ~(size_t)15;
+// Per-wave LDS queue of shadow connections waiting for a lane (see
+// trace_rays_refill): kQueueCap slots of kQueueFields words.
+constexpr int kQueueCap = 64;
+constexpr int kQueueFields = 7 + kBands + 2 + 3;  // origin, direction, tmax | energies | bin, ks | d0
+
+// LDS bytes private to the lanes and waves of one workgroup: the per-lane
+// stacks (rounded up to 16) followed by the per-wave connection queues, so
+// that the float4 node staging after them stays aligned.
+__host__ __device__ constexpr size_t stack_only_bytes(int wg) {
+    return ((size_t)kStack * wg * sizeof(unsigned short) + 15) & ~(size_t)15;
 }
-inline size_t stack_bytes(int wg) { return ((size_t)kStack * wg * sizeof(unsigned short) + 15) & ~(size_t)15; }
+__host__ __device__ constexpr size_t private_bytes(int wg) {
+    return stack_only_bytes(wg) + (size_t)(wg / 64) * kQueueFields * kQueueCap * sizeof(float);
+}
 
 // A device-resident triangle scene: 4-wide BVH nodes (8 x float4 each),
 // triangles in leaf order (3 x float4: v0|tri id, e1|surface id, e2|0),
@@ -86,6 +95,7 @@ struct LaneStats {
 // LDS views of one workgroup.
 struct Lds {
     unsigned short* stack;  // kStack x WG
+    float* queue;           // per wave: kQueueFields x kQueueCap
     const float4* nodes; // staged top nodes
     float* hist;         // [Ks,6,K] or null
     int* hits;           // [S] or null (single source only)
@@ -196,162 +206,6 @@ __device__ __forceinline__ void slab4(const float4* v, const Ray& r, float tlim,
         const float tmax = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tlim));
         tn[c] = tmin;
         hit[c] = tmin <= tmax && tmax > kEps;
-    }
-}
-
-__device__ __forceinline__ void test_tri(const SceneView& sv, int k, const Ray& r, Closest& h) {
-    const float4* t4 = sv.tris + (size_t)k * 3;
-    const float4 ta = t4[0], tb = t4[1], tc = t4[2];
-    float t;
-    if (tri_test(ta, tb, tc, r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, t)) {
-        const int tri = __float_as_int(ta.w), surf = __float_as_int(tb.w);
-        if (better(t, surf, tri, h)) {
-            h.t = t;
-            h.surf = surf;
-            h.tri = tri;
-        }
-    }
-}
-
-template <int WG>
-__device__ void closest_hit(const SceneView& sv, const float4* s_nodes, unsigned short* stk, const Ray& r, Closest& h,
-                            unsigned& node_cnt, unsigned& tri_cnt, int* error_flag) {
-    h.t = kBig;
-    h.surf = -1;
-    h.tri = -1;
-    if (!sv.use_bvh) {
-        for (int k = 0; k < sv.num_tris; ++k) {
-            ++tri_cnt;
-            test_tri(sv, k, r, h);
-        }
-        return;
-    }
-    int sp = 0;
-    int node = 0;
-    while (true) {
-        ++node_cnt;
-        float4 v[7];
-        load_node(sv, s_nodes, node, v);
-        float tn[4];
-        bool hit[4];
-        slab4(v, r, h.t + kBvhEps, tn, hit);
-        const int ch[4] = {__float_as_int(v[6].x), __float_as_int(v[6].y), __float_as_int(v[6].z),
-                           __float_as_int(v[6].w)};
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (hit[c] && ch[c] <= -2) {
-                const int code = -(ch[c] + 2);
-                const int start = code >> 3, count = (code & 7) + 1;
-                for (int k = 0; k < count; ++k) {
-                    ++tri_cnt;
-                    test_tri(sv, start + k, r, h);
-                }
-            }
-        }
-        // internal children, nearest first
-        float key[4];
-        int idx[4];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const bool ok = hit[c] && ch[c] >= 0 && tn[c] <= h.t + kBvhEps;
-            key[c] = ok ? tn[c] : INFINITY;
-            idx[c] = ch[c];
-        }
-#define AA_CSWAP(i, j)         \
-    if (key[j] < key[i]) {     \
-        const float tk = key[i]; \
-        key[i] = key[j];       \
-        key[j] = tk;           \
-        const int ti = idx[i]; \
-        idx[i] = idx[j];       \
-        idx[j] = ti;           \
-    }
-        AA_CSWAP(0, 1)
-        AA_CSWAP(2, 3)
-        AA_CSWAP(0, 2)
-        AA_CSWAP(1, 3)
-        AA_CSWAP(1, 2)
-#undef AA_CSWAP
-#pragma unroll
-        for (int c = 3; c >= 1; --c) {
-            if (key[c] < INFINITY) {
-                if (sp < kStack) {
-                    stk[sp * WG] = (unsigned short)idx[c];
-                    ++sp;
-                } else {
-                    atomicOr(error_flag, 1);
-                }
-            }
-        }
-        if (key[0] < INFINITY) {
-            node = idx[0];
-        } else if (sp > 0) {
-            --sp;
-            node = (int)stk[sp * WG];
-        } else {
-            break;
-        }
-    }
-}
-
-template <int WG>
-__device__ bool any_hit(const SceneView& sv, const float4* s_nodes, unsigned short* stk, const Ray& r, float max_distance,
-                        unsigned& node_cnt, unsigned& tri_cnt, int* error_flag) {
-    const float lo = kEps, hi = max_distance - kEps;
-    if (!sv.use_bvh) {
-        for (int k = 0; k < sv.num_tris; ++k) {
-            const float4* t4 = sv.tris + (size_t)k * 3;
-            ++tri_cnt;
-            float t;
-            if (tri_test(t4[0], t4[1], t4[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, t) && t > lo && t < hi) return true;
-        }
-        return false;
-    }
-    int sp = 0;
-    int node = 0;
-    const float tlim = max_distance + kBvhEps;
-    while (true) {
-        ++node_cnt;
-        float4 v[7];
-        load_node(sv, s_nodes, node, v);
-        float tn[4];
-        bool hit[4];
-        slab4(v, r, tlim, tn, hit);
-        const int ch[4] = {__float_as_int(v[6].x), __float_as_int(v[6].y), __float_as_int(v[6].z),
-                           __float_as_int(v[6].w)};
-        int next = -1;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            if (!hit[c]) continue;
-            if (ch[c] <= -2) {
-                const int code = -(ch[c] + 2);
-                const int start = code >> 3, count = (code & 7) + 1;
-                for (int k = 0; k < count; ++k) {
-                    const float4* t4 = sv.tris + (size_t)(start + k) * 3;
-                    ++tri_cnt;
-                    float t;
-                    if (tri_test(t4[0], t4[1], t4[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, t) && t > lo && t < hi)
-                        return true;
-                }
-            } else if (ch[c] >= 0) {
-                if (next < 0) {
-                    next = ch[c];
-                } else if (sp < kStack) {
-                    stk[sp * WG] = (unsigned short)ch[c];
-                    ++sp;
-                } else {
-                    atomicOr(error_flag, 1);
-                }
-            }
-        }
-        if (next >= 0) {
-            node = next;
-        } else if (sp > 0) {
-            --sp;
-            node = (int)stk[sp * WG];
-        } else {
-            return false;
-        }
     }
 }
 
@@ -491,107 +345,211 @@ __device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out
     return true;
 }
 
-// One bounce of the reference loop (steam_rt.py:4743-4848): closest hit,
-// termination tests, next-event deposit per live source, continuation.
-// Returns true if the ray needs another bounce; otherwise it has been
-// finished.  LDS_HIST: histogram privatised in lds.hist, else global atomics
-// into out.echogram.  lds.hits != null: single-source per-surface tallies in
-// LDS.
-template <int WG, bool LDS_HIST>
-__device__ bool step_ray(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds, RayState& s,
-                         LaneStats& st, int* error_flag) {
-    unsigned short* stk = lds.stack + threadIdx.x;
-    const int bounce = s.bounce;
-    const int K = p.num_bins;
+// ---------------------------------------------------------------------------
+// Per-lane state machine: one loop iteration = one traversal step
+// ---------------------------------------------------------------------------
+//
+// A lane is IDLE (needs a ray), in a BOUNCE query (closest hit of the ray's
+// current segment) or in a SHADOW query (any hit towards one source).  Every
+// iteration advances each busy lane's query by one BVH node (with its leaf
+// triangles), so the lanes of a wave run the same traversal code whatever
+// bounce or query they are in.  When a query ends, the lane does the
+// reference's work for that point of the bounce loop (steam_rt.py:4743-4848)
+// and sets up its next query:
+//   bounce done -> termination tests, hit counts, then the first shadow
+//                  query of the hit (sources in ascending order), or the
+//                  continuation if no source needs one;
+//   shadow done -> deposit if unoccluded, then the next source's shadow
+//                  query, or the continuation;
+//   continuation -> energy / path update, specular or diffuse direction, then
+//                  the next bounce query, or the end of the ray.
+// The arithmetic and its order are those of the one-ray-at-a-time loop.
+
+enum : int { kIdle = 0, kBounce = 1, kShadow = 2 };
+
+struct Query {
     Ray r;
-    r.ox = s.ox;
-    r.oy = s.oy;
-    r.oz = s.oz;
-    r.dx = s.dx;
-    r.dy = s.dy;
-    r.dz = s.dz;
-    r.ix = safe_inv(r.dx);
-    r.iy = safe_inv(r.dy);
-    r.iz = safe_inv(r.dz);
-    Closest h;
-    ++st.q_b;
-    closest_hit<WG>(sv, lds.nodes, stk, r, h, st.nb, st.tb, error_flag);
-    if (out.log_tri != nullptr && s.group == 0 && s.ray < out.log_rays && bounce < out.log_bounces) {
-        out.log_tri[(size_t)s.ray * out.log_bounces + bounce] = h.tri;
-        out.log_t[(size_t)s.ray * out.log_bounces + bounce] = h.t;
-    }
-    if (h.surf < 0) ++st.escaped;
-    if (h.surf < 0 || h.t <= p.listener_radius || s.accum > p.max_path_len) {
-        finish_ray(p, out, s, false, bounce + 1, st);
-        return false;
-    }
-    if (bounce > 0) {
-        if (sphere_before(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, p.lx, p.ly, p.lz, p.listener_radius, h.t)) {
-            finish_ray(p, out, s, false, bounce + 1, st);
-            return false;
-        }
-        unsigned long long m = s.mask;
-        while (m) {
-            const int k = __builtin_ctzll(m);
-            m &= m - 1ull;
-            const float4 S = p.sources[k];
-            const float sdist =
-                sqrtf((p.lx - S.x) * (p.lx - S.x) + (p.ly - S.y) * (p.ly - S.y) + (p.lz - S.z) * (p.lz - S.z));
-            if (sdist > p.source_radius &&
-                sphere_before(r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, S.x, S.y, S.z, p.source_radius, h.t)) {
-                s.mask &= ~(1ull << k);
-                atomicMax(&out.actual_bounces[k], bounce + 1);
+    float tmax;   // shadow: distance to the source
+    int node;     // next node to visit; -1 when the query is finished
+    int sp;       // stack depth
+    Closest h;    // bounce: best hit so far
+    bool occluded;
+};
+
+// Ray-tracing context of the lane between the queries of one bounce.
+struct HitContext {
+    float hx, hy, hz;        // offset hit point (shadow origin, next origin)
+    float nx, ny, nz;        // normal oriented against the ray
+    float t;                 // segment length
+    int surf;
+    unsigned long long nee;  // sources still to connect at this hit
+};
+
+__device__ __forceinline__ void begin_query(Query& q, float ox, float oy, float oz, float dx, float dy, float dz,
+                                            float tmax) {
+    q.r.ox = ox;
+    q.r.oy = oy;
+    q.r.oz = oz;
+    q.r.dx = dx;
+    q.r.dy = dy;
+    q.r.dz = dz;
+    q.r.ix = safe_inv(dx);
+    q.r.iy = safe_inv(dy);
+    q.r.iz = safe_inv(dz);
+    q.tmax = tmax;
+    q.node = 0;
+    q.sp = 0;
+    q.h.t = kBig;
+    q.h.surf = -1;
+    q.h.tri = -1;
+    q.occluded = false;
+}
+
+// Triangles [start, start + count) against the query: bounce queries keep
+// the closest hit, shadow queries stop at the first hit before the source
+// (tri_test already requires t > EPS, the lower end of (EPS, tmax - EPS)).
+// One copy serves both kinds, so lanes of either kind test together.
+__device__ __forceinline__ void test_leaf(const SceneView& sv, int start, int count, Query& q, bool shadow,
+                                          unsigned& tri_cnt) {
+    for (int k = 0; k < count; ++k) {
+        ++tri_cnt;
+        const float4* t4 = sv.tris + (size_t)(start + k) * 3;
+        const float4 ta = t4[0], tb = t4[1], tc = t4[2];
+        float t;
+        if (tri_test(ta, tb, tc, q.r.ox, q.r.oy, q.r.oz, q.r.dx, q.r.dy, q.r.dz, t)) {
+            if (shadow) {
+                if (t < q.tmax - kEps) {
+                    q.occluded = true;
+                    break;
+                }
+            } else {
+                const int tri = __float_as_int(ta.w), surf = __float_as_int(tb.w);
+                if (better(t, surf, tri, q.h)) {
+                    q.h.t = t;
+                    q.h.surf = surf;
+                    q.h.tri = tri;
+                }
             }
         }
-        if (s.mask == 0ull) return false;  // every source has recorded this ray's end
     }
-    const float4 nn = sv.normals[h.tri];
-    float nx = nn.x, ny = nn.y, nz = nn.z;
-    if (nx * r.dx + ny * r.dy + nz * r.dz > 0.0f) {
-        nx = -nx;
-        ny = -ny;
-        nz = -nz;
+}
+
+// One traversal step of `q`: visit one 4-wide node and the triangles of its
+// hit leaf children, then push the hit internal children nearest first and
+// move to the nearest (or pop).  Bounce queries cull boxes beyond the best
+// hit; shadow queries beyond the source and stop at the first occluder.
+template <int WG>
+__device__ __forceinline__ void traversal_step(const SceneView& sv, const float4* s_nodes, unsigned short* stk,
+                                               Query& q, bool shadow, unsigned& node_cnt, unsigned& tri_cnt,
+                                               int* error_flag) {
+    if (!sv.use_bvh) {
+        test_leaf(sv, 0, sv.num_tris, q, shadow, tri_cnt);
+        q.node = -1;
+        return;
     }
-    const int surf = h.surf;
-    const float t = h.t;
-    const float hx = r.ox + t * r.dx + kHitOffset * nx;
-    const float hy = r.oy + t * r.dy + kHitOffset * ny;
-    const float hz = r.oz + t * r.dz + kHitOffset * nz;
-    const float scat = sv.scattering[surf];
-    float refl[kBands];
+    ++node_cnt;
+    float4 v[7];
+    load_node(sv, s_nodes, q.node, v);
+    const float tlim = (shadow ? q.tmax : q.h.t) + kBvhEps;
+    float tn[4];
+    bool hit[4];
+    slab4(v, q.r, tlim, tn, hit);
+    const int ch[4] = {__float_as_int(v[6].x), __float_as_int(v[6].y), __float_as_int(v[6].z), __float_as_int(v[6].w)};
+    unsigned leaves = 0;
 #pragma unroll
-    for (int b = 0; b < kBands; ++b) refl[b] = sv.reflection[surf * kBands + b];
-    unsigned long long m = s.mask;
-    while (m) {
-        const int k = __builtin_ctzll(m);
-        m &= m - 1ull;
-        if (lds.hits)
-            atomicAdd(&lds.hits[surf], 1);
-        else
-            atomicAdd(&out.hit_counts[(size_t)k * sv.num_surfaces + surf], 1);
-        const float4 S = p.sources[k];
-        const float tsx = S.x - hx, tsy = S.y - hy, tsz = S.z - hz;
-        const float D = sqrtf(tsx * tsx + tsy * tsy + tsz * tsz);
-        if (!(D > p.dmin)) continue;
-        const float sdx = tsx / D, sdy = tsy / D, sdz = tsz / D;
-        const float facing = nx * sdx + ny * sdy + nz * sdz;
-        if (!(facing > 0.0f)) continue;
-        Ray sr;
-        sr.ox = hx;
-        sr.oy = hy;
-        sr.oz = hz;
-        sr.dx = sdx;
-        sr.dy = sdy;
-        sr.dz = sdz;
-        sr.ix = safe_inv(sdx);
-        sr.iy = safe_inv(sdy);
-        sr.iz = safe_inv(sdz);
-        ++st.q_s;
-        if (any_hit<WG>(sv, lds.nodes, stk, sr, D, st.ns, st.ts, error_flag)) continue;
+    for (int c = 0; c < 4; ++c) leaves |= (hit[c] && ch[c] <= -2) ? (1u << c) : 0u;
+    while (leaves && !q.occluded) {
+        const int c = __builtin_ctz(leaves);
+        leaves &= leaves - 1u;
+        const int code = -((c == 0 ? ch[0] : c == 1 ? ch[1] : c == 2 ? ch[2] : ch[3]) + 2);
+        test_leaf(sv, code >> 3, (code & 7) + 1, q, shadow, tri_cnt);
+    }
+    if (q.occluded) {
+        q.node = -1;
+        return;
+    }
+    const float cull = shadow ? tlim : q.h.t + kBvhEps;
+    float key[4];
+    int idx[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const bool ok = hit[c] && ch[c] >= 0 && tn[c] <= cull;
+        key[c] = ok ? tn[c] : INFINITY;
+        idx[c] = ch[c];
+    }
+#define AA_CSWAP(i, j)           \
+    if (key[j] < key[i]) {       \
+        const float tk = key[i]; \
+        key[i] = key[j];         \
+        key[j] = tk;             \
+        const int ti = idx[i];   \
+        idx[i] = idx[j];         \
+        idx[j] = ti;             \
+    }
+    AA_CSWAP(0, 1)
+    AA_CSWAP(2, 3)
+    AA_CSWAP(0, 2)
+    AA_CSWAP(1, 3)
+    AA_CSWAP(1, 2)
+#undef AA_CSWAP
+#pragma unroll
+    for (int c = 3; c >= 1; --c) {
+        if (key[c] < INFINITY) {
+            if (q.sp < kStack) {
+                stk[q.sp * WG] = (unsigned short)idx[c];
+                ++q.sp;
+            } else {
+                atomicOr(error_flag, 1);
+            }
+        }
+    }
+    if (key[0] < INFINITY) {
+        q.node = idx[0];
+    } else if (q.sp > 0) {
+        --q.sp;
+        q.node = (int)stk[q.sp * WG];
+    } else {
+        q.node = -1;
+    }
+}
+
+// A shadow connection from the current hit to one source, with its deposit
+// (steam_rt.py:4793-4826) evaluated before the shadow query: the query itself
+// is the lane's Query (origin = hit point, direction = towards the source,
+// tmax = distance); if the query finds no occluder the lane adds these
+// energies.  Evaluating the deposit first lets any lane of the wave resolve
+// the connection while the ray that made it goes on to its next bounce.
+struct Connection {
+    float e[kBands];      // energy per band
+    int bin;              // histogram bin
+    int ks;               // source | surface << 6
+    float d0x, d0y, d0z;  // initial direction of the ray (FOA weights)
+};
+
+// Connection from the hit in `c` to source k (steam_rt.py:4770-4826, same
+// arithmetic and order).  Returns false if the reference casts no shadow
+// query (D <= d_min or the source is behind the surface); bin is -1 when the
+// arrival falls outside the histogram (the query is counted, not traced: it
+// cannot change the result).
+__device__ __forceinline__ bool connect(const SceneView& sv, const RayParams& p, const RayState& s,
+                                        const HitContext& c, int k, Query& q, Connection& cn) {
+    const float4 S = p.sources[k];
+    const float tsx = S.x - c.hx, tsy = S.y - c.hy, tsz = S.z - c.hz;
+    const float D = sqrtf(tsx * tsx + tsy * tsy + tsz * tsz);
+    if (!(D > p.dmin)) return false;
+    const float sdx = tsx / D, sdy = tsy / D, sdz = tsz / D;
+    const float facing = c.nx * sdx + c.ny * sdy + c.nz * sdz;
+    if (!(facing > 0.0f)) return false;
+    const int K = p.num_bins;
+    const float rel_delay = (s.accum + c.t + D) / p.c - S.w;
+    const float qb = rel_delay / p.bin_dur;
+    cn.bin = (qb >= 0.0f && qb < (float)K) ? (int)floorf(qb) : -1;
+    if (cn.bin >= 0) {
+        const float scat = sv.scattering[c.surf];
         const float diffuse = 0.31830988618379067f * scat * facing;
-        float hvx = sdx - r.dx, hvy = sdy - r.dy, hvz = sdz - r.dz;
+        float hvx = sdx - s.dx, hvy = sdy - s.dy, hvz = sdz - s.dz;
         normalize3(hvx, hvy, hvz);
-        float cos_half = hvx * nx + hvy * ny + hvz * nz;
+        float cos_half = hvx * c.nx + hvy * c.ny + hvz * c.nz;
         cos_half = cos_half < 0.0f ? 0.0f : cos_half;
         const float specular =
             (p.spec_exp + 2.0f) * 0.039788735772973836f * (1.0f - scat) * powf(cos_half, p.spec_exp);
@@ -599,44 +557,117 @@ __device__ bool step_ray(const SceneView& sv, const RayParams& p, const Outputs&
         const float distance_term = 0.07957747154594767f / (dm * dm);
         const float source_cosine = -(p.fx * sdx + p.fy * sdy + p.fz * sdz);
         const float source_gain = powf(fabsf((1.0f - p.dip_w) + p.dip_w * source_cosine), p.dip_p);
-        const float rel_delay = (s.accum + t + D) / p.c - S.w;
-        const float q = rel_delay / p.bin_dur;
-        if (!(q >= 0.0f && q < (float)K)) continue;
-        const int bin = (int)floorf(q);
         const float base = p.ray_scale * source_gain * distance_term * (diffuse + specular);
-        float esum = 0.0f;
 #pragma unroll
-        for (int b = 0; b < kBands; ++b) {
-            const float e = base * refl[b] * s.A[b];
-            const size_t hi = ((size_t)k * kBands + b) * K + bin;
-            if (LDS_HIST)
-                atomicAdd(&lds.hist[hi], e);
-            else
-                atomicAdd(&out.echogram[hi], e);
-            if (p.amb) {
-                float* am = out.ambisonic + (((size_t)k * kBands + b) * 4) * K + bin;
-                atomicAdd(am, e);
-                atomicAdd(am + K, e * s.d0x);
-                atomicAdd(am + 2 * K, e * s.d0y);
-                atomicAdd(am + 3 * K, e * s.d0z);
-            }
-            esum += e;
-        }
-        ++st.contrib;
-        if (lds.hits) {
-            atomicAdd(&lds.contrib[surf], 1);
-            atomicAdd(&lds.senergy[surf], esum);
-        } else {
-            atomicAdd(&out.contrib_counts[(size_t)k * sv.num_surfaces + surf], 1);
-            atomicAdd(&out.surface_energy[(size_t)k * sv.num_surfaces + surf], esum);
-        }
+        for (int b = 0; b < kBands; ++b) cn.e[b] = base * sv.reflection[c.surf * kBands + b] * s.A[b];
+        cn.ks = k | (c.surf << 6);
+        cn.d0x = s.d0x;
+        cn.d0y = s.d0y;
+        cn.d0z = s.d0z;
+        begin_query(q, c.hx, c.hy, c.hz, sdx, sdy, sdz, D);
     }
+    return true;
+}
+
+// Next connection of the hit that needs a shadow query (sources in ascending
+// order); false once every source of the hit is resolved.
+__device__ __forceinline__ bool next_connection(const SceneView& sv, const RayParams& p, const RayState& s,
+                                                HitContext& c, Query& q, Connection& cn, LaneStats& st) {
+    while (c.nee) {
+        const int k = __builtin_ctzll(c.nee);
+        c.nee &= c.nee - 1ull;
+        if (!connect(sv, p, s, c, k, q, cn)) continue;
+        ++st.q_s;
+        if (cn.bin >= 0) return true;
+    }
+    return false;
+}
+
+// Deposit of an unoccluded connection (steam_rt.py:4810-4826).
+template <bool LDS_HIST>
+__device__ __forceinline__ void deposit(const RayParams& p, const Outputs& out, const Lds& lds, int num_surfaces,
+                                        const Connection& cn, LaneStats& st) {
+    const int K = p.num_bins;
+    const int k = cn.ks & 63, surf = cn.ks >> 6;
+    float esum = 0.0f;
 #pragma unroll
-    for (int b = 0; b < kBands; ++b) s.A[b] *= refl[b];
-    s.accum += t;
-    s.ox = hx;
-    s.oy = hy;
-    s.oz = hz;
+    for (int b = 0; b < kBands; ++b) {
+        const float e = cn.e[b];
+        const size_t hi = ((size_t)k * kBands + b) * K + cn.bin;
+        if (LDS_HIST)
+            atomicAdd(&lds.hist[hi], e);
+        else
+            atomicAdd(&out.echogram[hi], e);
+        if (p.amb) {
+            float* am = out.ambisonic + (((size_t)k * kBands + b) * 4) * K + cn.bin;
+            atomicAdd(am, e);
+            atomicAdd(am + K, e * cn.d0x);
+            atomicAdd(am + 2 * K, e * cn.d0y);
+            atomicAdd(am + 3 * K, e * cn.d0z);
+        }
+        esum += e;
+    }
+    ++st.contrib;
+    if (lds.hits) {
+        atomicAdd(&lds.contrib[surf], 1);
+        atomicAdd(&lds.senergy[surf], esum);
+    } else {
+        atomicAdd(&out.contrib_counts[(size_t)k * num_surfaces + surf], 1);
+        atomicAdd(&out.surface_energy[(size_t)k * num_surfaces + surf], esum);
+    }
+}
+
+// Field f of queue slot i is queue[f * kQueueCap + i] (consecutive lanes
+// touch consecutive words).
+
+__device__ __forceinline__ void park(float* qu, int slot, const Query& q, const Connection& cn) {
+    float* f = qu + slot;
+    f[0 * kQueueCap] = q.r.ox;
+    f[1 * kQueueCap] = q.r.oy;
+    f[2 * kQueueCap] = q.r.oz;
+    f[3 * kQueueCap] = q.r.dx;
+    f[4 * kQueueCap] = q.r.dy;
+    f[5 * kQueueCap] = q.r.dz;
+    f[6 * kQueueCap] = q.tmax;
+#pragma unroll
+    for (int b = 0; b < kBands; ++b) f[(7 + b) * kQueueCap] = cn.e[b];
+    f[(7 + kBands) * kQueueCap] = __int_as_float(cn.bin);
+    f[(8 + kBands) * kQueueCap] = __int_as_float(cn.ks);
+    f[(9 + kBands) * kQueueCap] = cn.d0x;
+    f[(10 + kBands) * kQueueCap] = cn.d0y;
+    f[(11 + kBands) * kQueueCap] = cn.d0z;
+}
+
+__device__ __forceinline__ void unpark(const float* qu, int slot, Query& q, Connection& cn) {
+    const float* f = qu + slot;
+    begin_query(q, f[0], f[1 * kQueueCap], f[2 * kQueueCap], f[3 * kQueueCap], f[4 * kQueueCap], f[5 * kQueueCap],
+                f[6 * kQueueCap]);
+#pragma unroll
+    for (int b = 0; b < kBands; ++b) cn.e[b] = f[(7 + b) * kQueueCap];
+    cn.bin = __float_as_int(f[(7 + kBands) * kQueueCap]);
+    cn.ks = __float_as_int(f[(8 + kBands) * kQueueCap]);
+    cn.d0x = f[(9 + kBands) * kQueueCap];
+    cn.d0y = f[(10 + kBands) * kQueueCap];
+    cn.d0z = f[(11 + kBands) * kQueueCap];
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Continuation after the hit's connections (steam_rt.py:4827-4848); returns
+// the lane's next phase (a new bounce query, or idle when the ray ended).
+__device__ __forceinline__ int continue_ray(const SceneView& sv, const RayParams& p, const Outputs& out,
+                                            RayState& s, const HitContext& c, Query& q, LaneStats& st) {
+    const int bounce = s.bounce;
+#pragma unroll
+    for (int b = 0; b < kBands; ++b) s.A[b] *= sv.reflection[c.surf * kBands + b];
+    s.accum += c.t;
+    s.ox = c.hx;
+    s.oy = c.hy;
+    s.oz = c.hz;
+    const float scat = sv.scattering[c.surf];
     float u;
     int j;
     if (p.rng_mode == AA_RNG_ARRAYS) {
@@ -648,6 +679,7 @@ __device__ bool step_ray(const SceneView& sv, const RayParams& p, const Outputs&
         u = (float)(w0 >> 8) * (1.0f / 16777216.0f);
         j = (int)(((unsigned long long)w1 * (unsigned long long)(unsigned)p.num_samples) >> 32);
     }
+    const float nx = c.nx, ny = c.ny, nz = c.nz;
     if (u < scat) {
         const float sx = p.bank[j * 3], sy = p.bank[j * 3 + 1], sz = p.bank[j * 3 + 2];
         float hx2 = 1.0f, hy2 = 0.0f, hz2 = 0.0f;
@@ -670,10 +702,10 @@ __device__ bool step_ray(const SceneView& sv, const RayParams& p, const Outputs&
         s.dy = ndy;
         s.dz = ndz;
     } else {
-        const float dot = r.dx * nx + r.dy * ny + r.dz * nz;
-        float ndx = r.dx - 2.0f * dot * nx;
-        float ndy = r.dy - 2.0f * dot * ny;
-        float ndz = r.dz - 2.0f * dot * nz;
+        const float dot = s.dx * nx + s.dy * ny + s.dz * nz;
+        float ndx = s.dx - 2.0f * dot * nx;
+        float ndy = s.dy - 2.0f * dot * ny;
+        float ndz = s.dz - 2.0f * dot * nz;
         normalize3(ndx, ndy, ndz);
         s.dx = ndx;
         s.dy = ndy;
@@ -681,18 +713,84 @@ __device__ bool step_ray(const SceneView& sv, const RayParams& p, const Outputs&
     }
     if (s.accum > p.max_path_len) {
         finish_ray(p, out, s, false, bounce + 1, st);
-        return false;
+        return kIdle;
     }
     s.bounce = bounce + 1;
     if (s.bounce >= p.num_bounces) {
         finish_ray(p, out, s, true, s.bounce, st);
+        return kIdle;
+    }
+    begin_query(q, s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, kBig);
+    ++st.q_b;
+    return kBounce;
+}
+
+// A finished bounce query (steam_rt.py:4744-4769): termination tests, hit
+// counts and the hit context.  Returns false if the ray has ended.
+__device__ __forceinline__ bool close_bounce(const SceneView& sv, const RayParams& p, const Outputs& out,
+                                            const Lds& lds, RayState& s, HitContext& c, const Query& q,
+                                            LaneStats& st) {
+    const int bounce = s.bounce;
+    const Closest h = q.h;
+    if (out.log_tri != nullptr && s.group == 0 && s.ray < out.log_rays && bounce < out.log_bounces) {
+        out.log_tri[(size_t)s.ray * out.log_bounces + bounce] = h.tri;
+        out.log_t[(size_t)s.ray * out.log_bounces + bounce] = h.t;
+    }
+    if (h.surf < 0) ++st.escaped;
+    if (h.surf < 0 || h.t <= p.listener_radius || s.accum > p.max_path_len) {
+        finish_ray(p, out, s, false, bounce + 1, st);
         return false;
     }
+    if (bounce > 0) {
+        if (sphere_before(s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, p.lx, p.ly, p.lz, p.listener_radius, h.t)) {
+            finish_ray(p, out, s, false, bounce + 1, st);
+            return false;
+        }
+        unsigned long long m = s.mask;
+        while (m) {
+            const int k = __builtin_ctzll(m);
+            m &= m - 1ull;
+            const float4 S = p.sources[k];
+            const float sdist =
+                sqrtf((p.lx - S.x) * (p.lx - S.x) + (p.ly - S.y) * (p.ly - S.y) + (p.lz - S.z) * (p.lz - S.z));
+            if (sdist > p.source_radius &&
+                sphere_before(s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, S.x, S.y, S.z, p.source_radius, h.t)) {
+                s.mask &= ~(1ull << k);
+                atomicMax(&out.actual_bounces[k], bounce + 1);
+            }
+        }
+        if (s.mask == 0ull) return false;  // every source has recorded this ray's end
+    }
+    const float4 nn = sv.normals[h.tri];
+    float nx = nn.x, ny = nn.y, nz = nn.z;
+    if (nx * s.dx + ny * s.dy + nz * s.dz > 0.0f) {
+        nx = -nx;
+        ny = -ny;
+        nz = -nz;
+    }
+    c.surf = h.surf;
+    c.t = h.t;
+    c.nx = nx;
+    c.ny = ny;
+    c.nz = nz;
+    c.hx = s.ox + h.t * s.dx + kHitOffset * nx;
+    c.hy = s.oy + h.t * s.dy + kHitOffset * ny;
+    c.hz = s.oz + h.t * s.dz + kHitOffset * nz;
+    unsigned long long m = s.mask;
+    while (m) {
+        const int k = __builtin_ctzll(m);
+        m &= m - 1ull;
+        if (lds.hits)
+            atomicAdd(&lds.hits[c.surf], 1);
+        else
+            atomicAdd(&out.hit_counts[(size_t)k * sv.num_surfaces + c.surf], 1);
+    }
+    c.nee = s.mask;
     return true;
 }
 
-// Hand out rays to the lanes of this wave that need one: one atomic per wave
-// on `cursor`; lanes past `limit` get -1.
+// Hand out tasks to the lanes of this wave that need one: one atomic per
+// wave on `cursor`; lanes past `limit` get -1.
 __device__ __forceinline__ int claim_rays(bool need, unsigned* cursor, int limit) {
     const unsigned long long ballot = __ballot(need);
     if (ballot == 0ull) return -1;
@@ -708,36 +806,118 @@ __device__ __forceinline__ int claim_rays(bool need, unsigned* cursor, int limit
     return idx < (unsigned)limit ? (int)idx : -1;
 }
 
-// All rays of [first, first + count) handed out by `cursor` (relative to
-// `first`), traced by the calling wave's lanes with refill.
+// A wave runs its lanes' query completions (shading, connections, refill) in
+// one pass once at least 1/kFinishDiv of its lanes wait for one, so that the
+// completion code runs with many lanes and the traversal steps between passes
+// with few waiting; with fewer live lanes than this (end of the work pool) a
+// lane that waits is served at once.
+constexpr int kFinishDiv = 4;
+
+// Trace the tasks [first, first + count) handed out by `cursor` (relative to
+// `first`) with the calling wave's lanes.  A lane holds a ray (bounce queries,
+// and its own connections when the queue is full) or a parked connection of
+// another lane's ray; idle lanes take parked connections first, then new rays.
 template <int WG, bool LDS_HIST>
 __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds,
                                   unsigned* cursor, int first, int count, LaneStats& st, int* error_flag) {
+    unsigned short* stk = lds.stack + threadIdx.x;
+    float* qu = lds.queue + (size_t)(threadIdx.x >> 6) * (kQueueFields * kQueueCap);
+    const int lane = (int)(threadIdx.x & 63);
+    const unsigned long long below = (1ull << lane) - 1ull;
     RayState s;
-    bool has = false;
+    HitContext c;
+    Query q;
+    Connection cn;
+    q.node = -1;
+    q.occluded = false;
+    int phase = kIdle;
+    bool own = false;        // kShadow: the connection is this lane's ray's, which waits for it
+    int q_head = 0, q_count = 0;  // wave-uniform ring state of the parked connections
     bool exhausted = false;  // wave-uniform: a claim came back empty, so the pool is drained
     while (true) {
-        if (!exhausted) {
-            const bool need = !has;
-            const int got = claim_rays(need, cursor, count);
-            if (need && got >= 0) has = start_ray(p, out, first + got, s, st);
-            exhausted = __any(need && got < 0) != 0;
+        bool walking = phase != kIdle && q.node >= 0;
+        const int n_act = __popcll(__ballot(phase != kIdle));
+        const int n_trav = __popcll(__ballot(walking));
+        const bool drained = exhausted && q_count == 0;  // nothing left for idle lanes
+        const int n_wait = (n_act - n_trav) + (drained ? 0 : 64 - n_act);
+        if (n_wait == 0 && n_trav == 0) break;
+        if (n_wait > 0 && (n_trav == 0 || n_wait * kFinishDiv >= (drained ? n_act : 64))) {
+            bool nee = false;  // the lane's ray has connections of its current hit to resolve
+            if (phase != kIdle && !walking) {
+                if (phase == kBounce) {
+                    nee = close_bounce(sv, p, out, lds, s, c, q, st);
+                    if (!nee) phase = kIdle;
+                } else {
+                    if (!q.occluded) deposit<LDS_HIST>(p, out, lds, sv.num_surfaces, cn, st);
+                    nee = own;
+                    if (!own) phase = kIdle;
+                }
+            }
+            // park the new connections while the queue has room; a lane whose
+            // connection does not fit resolves it itself before going on
+            while (__any(nee)) {
+                bool has = false;
+                if (nee) {
+                    has = next_connection(sv, p, s, c, q, cn, st);
+                    if (!has) {
+                        nee = false;
+                        phase = continue_ray(sv, p, out, s, c, q, st);
+                    }
+                }
+                const unsigned long long want = __ballot(has);
+                const int room = kQueueCap - q_count;
+                if (has) {
+                    const int rank = __popcll(want & below);
+                    if (rank < room) {
+                        park(qu, (q_head + q_count + rank) & (kQueueCap - 1), q, cn);
+                    } else {
+                        phase = kShadow;
+                        own = true;
+                        nee = false;
+                    }
+                }
+                q_count += min(__popcll(want), room);
+            }
+            if (q_count > 0) {
+                wave_sync();  // parked connections are visible to every lane of the wave
+                const unsigned long long idle = __ballot(phase == kIdle);
+                const int take = min(__popcll(idle), q_count);
+                if (phase == kIdle) {
+                    const int rank = __popcll(idle & below);
+                    if (rank < take) {
+                        unpark(qu, (q_head + rank) & (kQueueCap - 1), q, cn);
+                        phase = kShadow;
+                        own = false;
+                    }
+                }
+                q_head = (q_head + take) & (kQueueCap - 1);
+                q_count -= take;
+                wave_sync();  // slots are read before later passes reuse them
+            }
+            if (!exhausted) {
+                const bool need = phase == kIdle;
+                const int got = claim_rays(need, cursor, count);
+                if (need && got >= 0 && start_ray(p, out, first + got, s, st)) {
+                    begin_query(q, s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, kBig);
+                    ++st.q_b;
+                    phase = kBounce;
+                }
+                exhausted = __any(need && got < 0) != 0;
+            }
+            walking = phase != kIdle && q.node >= 0;
         }
-        if (!__any(has)) {
-            if (exhausted) break;
-            continue;  // every lane's new ray had no bounce to trace; claim again
+        if (walking) {
+            const bool shadow = phase == kShadow;
+            unsigned nodes = 0, tris = 0;
+            traversal_step<WG>(sv, lds.nodes, stk, q, shadow, nodes, tris, error_flag);
+            if (shadow) {
+                st.ns += nodes;
+                st.ts += tris;
+            } else {
+                st.nb += nodes;
+                st.tb += tris;
+            }
         }
-        if (has) has = step_ray<WG, LDS_HIST>(sv, p, out, lds, s, st, error_flag);
-    }
-}
-
-// Trace one ray to the end (used where no refill is wanted).
-template <int WG, bool LDS_HIST>
-__device__ void trace_one_ray(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds, int ray,
-                              LaneStats& st, int* error_flag) {
-    RayState s;
-    if (!start_ray(p, out, ray, s, st)) return;
-    while (step_ray<WG, LDS_HIST>(sv, p, out, lds, s, st, error_flag)) {
     }
 }
 

@@ -72,13 +72,16 @@ __global__ void fill_zero_kernel(unsigned* p, size_t words) {
         p[i] = 0u;
 }
 
-// Persistent lanes over the rays of one trace (refilled as rays end).
+// Persistent lanes over the rays of one trace (refilled as rays end).  The
+// register budget is held to 3 waves per SIMD (168 VGPRs, no spills): the
+// tracer needs waves in flight to hide its dependent loads.
 template <int WG, bool LDS_HIST>
-__global__ __launch_bounds__(WG) void trace_kernel(const TraceArgs a) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void trace_kernel(const TraceArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int S = a.sv.num_surfaces;
     unsigned short* s_stack = reinterpret_cast<unsigned short*>(smem);
-    float4* s_nodes = reinterpret_cast<float4*>(smem + kStackBytes<WG>());
+    float* s_queue = reinterpret_cast<float*>(smem + stack_only_bytes(WG));
+    float4* s_nodes = reinterpret_cast<float4*>(smem + private_bytes(WG));
     float* s_hist = reinterpret_cast<float*>(s_nodes + (size_t)a.sv.lds_nodes * 8);
     const int hist_words = LDS_HIST ? a.p.num_sources * kBands * a.p.num_bins : 0;
     int* s_hits = reinterpret_cast<int*>(s_hist + hist_words);
@@ -100,6 +103,7 @@ __global__ __launch_bounds__(WG) void trace_kernel(const TraceArgs a) {
 
     Lds lds;
     lds.stack = s_stack;
+    lds.queue = s_queue;
     lds.nodes = s_nodes;
     lds.hist = LDS_HIST ? s_hist : nullptr;
     lds.hits = a.lds_surf ? s_hits : nullptr;
@@ -480,13 +484,15 @@ extern "C" int aa_trace(aa_scene_t scene, const aa_trace_params* p, const float*
         for (size_t l = 0; l < scene->wide.level_end.size(); ++l)
             if ((size_t)scene->wide.level_end[l] * sizeof(aa::WideNode) <= 4096) lds_nodes = scene->wide.level_end[l];
     }
-    lds_nodes = std::min(lds_nodes, num_nodes);
-    if (!p->use_bvh) lds_nodes = 0;
-    const size_t stack_bytes = aa::stack_bytes(wg);
-    const size_t node_bytes = (size_t)lds_nodes * sizeof(aa::WideNode);
+    const size_t stack_bytes = aa::private_bytes(wg);  // stacks + connection queues
     const size_t hist_bytes = (size_t)Ks * AA_NUM_BANDS * K * sizeof(float);
     const size_t surf_bytes = (size_t)S * 3 * sizeof(int);
     const size_t lds_cap = props.lds_per_block > 0 ? props.lds_per_block : 65536;
+    if (stack_bytes > lds_cap) return fail(AA_ERR_LIMIT, "LDS stacks of %zu bytes exceed %zu", stack_bytes, lds_cap);
+    // staged nodes only speed traversal up: as many as asked for that fit
+    lds_nodes = std::min({lds_nodes, num_nodes, (int)((lds_cap - stack_bytes) / sizeof(aa::WideNode))});
+    if (!p->use_bvh) lds_nodes = 0;
+    const size_t node_bytes = (size_t)lds_nodes * sizeof(aa::WideNode);
     bool lds_hist;
     if (p->lds_histogram >= 0)
         lds_hist = p->lds_histogram != 0;

@@ -190,7 +190,7 @@ typedef struct {
     int32_t block_size;         /* 64, 128, 256 or 512 lanes per workgroup */
     int32_t packet_rays;        /* rays per work item */
     int32_t lds_nodes;          /* top 4-wide nodes staged in LDS per scene */
-    int32_t lds_histogram;      /* 1: privatise each item's histogram in LDS */
+    int32_t lds_histogram;      /* ignored: batch histograms use global atomics (kept for layout) */
     int32_t workgroups_per_cu;  /* persistent grid = CUs x this (0: from LDS use) */
     int32_t use_graph;          /* 1: capture copy + zero + kernel in a hipGraph and replay it */
 } aa_batch_tuning;
