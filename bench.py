@@ -172,6 +172,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget-s", type=float, default=25.0, help="bound on the CPU baseline's wall time")
     ap.add_argument("--tuning", type=str, default="", help="JSON dict of launch overrides")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device-to-device three times (calibrates rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE against known bytes) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build (traces are independent "
@@ -180,6 +183,17 @@ def main() -> int:
 
     import torch
 
+    if args.copy_probe:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe": {"bytes_read": src.numel() * 4, "bytes_written": src.numel() * 4,
+                                         "copies": 3}}), flush=True)
+        return 0
     if not hip_backend.hip_available(0):
         print("no HIP device", file=sys.stderr)
         return 1
