@@ -8,7 +8,7 @@ primitives (walls, floor/ceiling polygon, boxes) or their triangle
 tessellation.  Only tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg may import it, and only as the checker.
 
-Pins (see tests/test_oracle_*.py):
+Pins (see tests/test_oracle.py):
   primitives    analytic known answers (exact t in a shoebox, boxes from
                 inside/outside, L-polygon inclusion, parallel thresholds)
   BVH           ao_build_bvh vs hand-checked trees; BVH == linear closest hit
