@@ -142,6 +142,8 @@ _SIGNATURES = {
     "aa_scene_info": (_I32, [_P, ctypes.POINTER(SceneInfo)]),
     "aa_scene_release": (_I32, [_P]),
     "aa_trace": (_I32, [_P, ctypes.POINTER(TraceParams), _P, _P, _P, _P, _P, ctypes.POINTER(TraceOutputs), _P]),
+    "aa_trace_plain": (_I32, [_P, ctypes.POINTER(TraceParams), _P, _P, _P, _P, _P, ctypes.POINTER(TraceOutputs),
+                              _P]),
     "aa_stream_sync": (_I32, [_P]),
     "aa_batch_create": (_I32, [_I32, _I32, ctypes.POINTER(_P)]),
     "aa_batch_submit": (_I32, [_P, ctypes.POINTER(BatchTrace), _I32, ctypes.POINTER(BatchTuning), _P]),

@@ -16,7 +16,7 @@ import json
 import math
 import threading
 from pathlib import Path
-from typing import Any, Sequence
+from typing import Any, Callable, Sequence
 
 import numpy as np
 
@@ -72,8 +72,11 @@ class Autotuner:
 
     def tune_batch(self, requests: Sequence[TraceRequest], *, space: dict[str, Sequence[int]] | None = None,
                    sample: int = 48, reps: int = 2, engine: BatchEngine | None = None,
-                   force: bool = False) -> BatchTuning:
-        """Best BatchTuning for batches like ``requests`` (timed on a sample)."""
+                   force: bool = False, progress: Callable[[str], None] | None = None) -> BatchTuning:
+        """Best BatchTuning for batches like ``requests`` (timed on a sample).
+
+        ``progress`` receives one line per timed candidate.
+        """
         import torch
 
         tri = int(np.median([r.scene.info["num_triangles"] for r in requests]))
@@ -102,6 +105,8 @@ class Autotuner:
                 times.append(a.elapsed_time(b))
             t = float(np.median(times))
             self.log.append(("batch:" + cls, cand, t))
+            if progress is not None:
+                progress(f"autotune batch {cls} {cand}: {t:.2f} ms")
             if t < best_t:
                 best, best_t = cand, t
         if own:

@@ -8,10 +8,10 @@ flat device arenas (no per-trace allocation), sources go up in one copy, and
 the persistent kernel pulls ray packets from a device work queue.  Traces are
 ordered largest first so the queue drains evenly.
 
-``trace_sequential`` is the plain one-scene-per-launch path (one ``aa_trace``
-per trace, synchronised and copied back per call, as the reference's backend
-does per ``trace_energy_field`` call); it is the baseline the batched path is
-measured against.
+``trace_sequential`` is the one-scene-per-launch path (one ``aa_trace`` or
+``aa_trace_plain`` per trace, synchronised and copied back per call, as the
+reference's backend does per ``trace_energy_field`` call); it is the baseline
+the batched path is measured against.
 """
 from __future__ import annotations
 
@@ -248,8 +248,13 @@ class BatchEngine:
         return BatchResult(reqs, lay, (f32, i32, i64, src), event, self.device)
 
 
-def trace_sequential(requests: Sequence[TraceRequest], stream=None) -> tuple[list[dict[str, Any]], float]:
-    """Plain path: one launch per trace, synchronised and copied back per call."""
+def trace_sequential(requests: Sequence[TraceRequest], stream=None,
+                     plain: bool = False) -> tuple[list[dict[str, Any]], float]:
+    """One launch per trace, synchronised and copied back per call.
+
+    ``plain`` uses the one-thread-per-ray kernel of the reference GPU backend
+    (``aa_trace_plain``), else the persistent single-trace kernel.
+    """
     import torch
 
     out = []
@@ -261,7 +266,7 @@ def trace_sequential(requests: Sequence[TraceRequest], stream=None) -> tuple[lis
         buf = TraceBuffers.allocate(r.scene.device, r.num_sources, r.num_bins, r.scene.num_surfaces,
                                     r.render_ambisonics)
         s = stream if stream is not None else torch.cuda.current_stream(r.scene.device)
-        launch_trace(r.scene, r.params(), src, bank, buf, stream=s)
+        launch_trace(r.scene, r.params(), src, bank, buf, stream=s, plain=plain)
         s.synchronize()
         out.append({
             "echogram": buf.echogram.cpu().numpy(),

@@ -486,11 +486,13 @@ __device__ __forceinline__ void traversal_step(const SceneView& sv, const float4
         idx[i] = idx[j];         \
         idx[j] = ti;             \
     }
-    AA_CSWAP(0, 1)
-    AA_CSWAP(2, 3)
-    AA_CSWAP(0, 2)
-    AA_CSWAP(1, 3)
-    AA_CSWAP(1, 2)
+    if (!shadow) {  // any-hit queries need no order (skipped wave-wide when no lane is in a bounce query)
+        AA_CSWAP(0, 1)
+        AA_CSWAP(2, 3)
+        AA_CSWAP(0, 2)
+        AA_CSWAP(1, 3)
+        AA_CSWAP(1, 2)
+    }
 #undef AA_CSWAP
 #pragma unroll
     for (int c = 3; c >= 1; --c) {
@@ -555,8 +557,12 @@ __device__ __forceinline__ bool connect(const SceneView& sv, const RayParams& p,
             (p.spec_exp + 2.0f) * 0.039788735772973836f * (1.0f - scat) * powf(cos_half, p.spec_exp);
         const float dm = D > p.dmin ? D : p.dmin;
         const float distance_term = 0.07957747154594767f / (dm * dm);
-        const float source_cosine = -(p.fx * sdx + p.fy * sdy + p.fz * sdz);
-        const float source_gain = powf(fabsf((1.0f - p.dip_w) + p.dip_w * source_cosine), p.dip_p);
+        // omni source (w = 0): (1 - 0) + 0 * cos = 1 exactly, and 1^p = 1
+        float source_gain = 1.0f;
+        if (p.dip_w != 0.0f) {
+            const float source_cosine = -(p.fx * sdx + p.fy * sdy + p.fz * sdz);
+            source_gain = powf(fabsf((1.0f - p.dip_w) + p.dip_w * source_cosine), p.dip_p);
+        }
         const float base = p.ray_scale * source_gain * distance_term * (diffuse + specular);
 #pragma unroll
         for (int b = 0; b < kBands; ++b) cn.e[b] = base * sv.reflection[c.surf * kBands + b] * s.A[b];
@@ -807,11 +813,13 @@ __device__ __forceinline__ int claim_rays(bool need, unsigned* cursor, int limit
 }
 
 // A wave runs its lanes' query completions (shading, connections, refill) in
-// one pass once at least 1/kFinishDiv of its lanes wait for one, so that the
-// completion code runs with many lanes and the traversal steps between passes
-// with few waiting; with fewer live lanes than this (end of the work pool) a
-// lane that waits is served at once.
-constexpr int kFinishDiv = 4;
+// one pass once at least kFinishEighths/8 of its lanes wait for one (of the
+// live lanes once the work pool is drained), or when no lane traverses.  The
+// completion code is long and runs with only the waiting lanes, so it pays to
+// gather most of the wave first; measured on MI355X over 2/8..8/8, 7/8 gave
+// the shortest traces (8/8, pure lockstep, lets the slowest lane hold the
+// wave).
+constexpr int kFinishEighths = 7;
 
 // Trace the tasks [first, first + count) handed out by `cursor` (relative to
 // `first`) with the calling wave's lanes.  A lane holds a ray (bounce queries,
@@ -841,7 +849,7 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
         const bool drained = exhausted && q_count == 0;  // nothing left for idle lanes
         const int n_wait = (n_act - n_trav) + (drained ? 0 : 64 - n_act);
         if (n_wait == 0 && n_trav == 0) break;
-        if (n_wait > 0 && (n_trav == 0 || n_wait * kFinishDiv >= (drained ? n_act : 64))) {
+        if (n_wait > 0 && (n_trav == 0 || n_wait * 8 >= (drained ? n_act : 64) * kFinishEighths)) {
             bool nee = false;  // the lane's ray has connections of its current hit to resolve
             if (phase != kIdle && !walking) {
                 if (phase == kBounce) {

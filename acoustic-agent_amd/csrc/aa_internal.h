@@ -87,6 +87,12 @@ struct aa_scene_s {
     aa::WideBVH wide;
     float* d_nodes = nullptr;      // [nodes, 8, 4]
     float* d_tris = nullptr;       // [T, 3, 4] leaf order
+    // canonical binary tree for the plain one-thread-per-ray kernel:
+    // per node (lo xyz | escape) (hi xyz | start << 4 | count); triangles in
+    // its primitive order, same 3 x float4 format as d_tris
+    float* d_plain_nodes = nullptr;
+    float* d_plain_tris = nullptr;
+    int32_t plain_node_count = 0;
     float* d_normals = nullptr;    // [T, 4]
     float* d_reflection = nullptr; // [S, 6]
     float* d_scattering = nullptr; // [S]

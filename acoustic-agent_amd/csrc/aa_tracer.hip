@@ -131,6 +131,156 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void tr
     }
 }
 
+// ---------------------------------------------------------------------------
+// plain kernel: the reference GPU backend's execution model on gfx950
+// ---------------------------------------------------------------------------
+//
+// One thread per lane task, no persistence and no refill, blocks of 128
+// (steam_rt_cuda.py:491-710): each thread walks the canonical median-split
+// binary tree stacklessly with escape links (steam_rt_cuda.py:392-456) for
+// its bounce queries and, like the reference, answers each shadow query with
+// a full closest-hit walk tested against (EPS, D - EPS) (steam_rt_cuda.py:
+// 636-661).  The bounce arithmetic is the shared one of aa_device.h, so the
+// results equal the persistent kernel's; this kernel is the plain
+// one-scene-per-launch baseline the persistent and batched paths are
+// measured against.
+
+struct PlainArgs {
+    SceneView sv;
+    const float4* nodes;  // [nodes, 2]: (lo | escape) (hi | start << 4 | count)
+    const float4* tris;   // canonical primitive order
+    int num_nodes;
+    RayParams p;
+    Outputs out;
+};
+
+// Ray-box test of the reference GPU path (steam_rt_cuda.py:392-411).
+__device__ __forceinline__ bool plain_box(const Ray& r, const float4 lo, const float4 hi, float max_distance) {
+    float enter = 0.0f, exit_ = max_distance + kBvhEps;
+    const float o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
+    const float l[3] = {lo.x, lo.y, lo.z}, u[3] = {hi.x, hi.y, hi.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (fabsf(d[a]) <= 1e-12f) {
+            if (o[a] < l[a] || o[a] > u[a]) return false;
+            continue;
+        }
+        float first = (l[a] - o[a]) / d[a], second = (u[a] - o[a]) / d[a];
+        if (first > second) {
+            const float t = first;
+            first = second;
+            second = t;
+        }
+        if (first > enter) enter = first;
+        if (second < exit_) exit_ = second;
+        if (enter > exit_) return false;
+    }
+    return exit_ > kEps;
+}
+
+// Closest hit by the stackless walk (steam_rt_cuda.py:429-449).
+__device__ Closest plain_closest(const PlainArgs& a, const Ray& r, unsigned& node_cnt, unsigned& tri_cnt) {
+    Closest h;
+    h.t = kBig;
+    h.surf = -1;
+    h.tri = -1;
+    // linear mode tests every triangle (steam_rt_cuda.py:415-426); the tie
+    // rule makes the order irrelevant
+    const int num_nodes = a.sv.use_bvh ? a.num_nodes : 1;
+    int node = 0;
+    while (node < num_nodes) {
+        ++node_cnt;
+        const float4 lo = a.nodes[2 * node], hi = a.nodes[2 * node + 1];
+        if (!a.sv.use_bvh) {
+            for (int k = 0; k < a.sv.num_tris; ++k) {
+                ++tri_cnt;
+                const float4* t4 = a.tris + (size_t)k * 3;
+                const float4 ta = t4[0], tb = t4[1], tc = t4[2];
+                float t;
+                if (tri_test(ta, tb, tc, r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, t)) {
+                    const int tri = __float_as_int(ta.w), surf = __float_as_int(tb.w);
+                    if (better(t, surf, tri, h)) {
+                        h.t = t;
+                        h.surf = surf;
+                        h.tri = tri;
+                    }
+                }
+            }
+            break;
+        }
+        if (!plain_box(r, lo, hi, h.t)) {
+            node = __float_as_int(lo.w);
+            continue;
+        }
+        const int sc = __float_as_int(hi.w);
+        const int count = sc & 15;
+        for (int k = 0; k < count; ++k) {
+            ++tri_cnt;
+            const float4* t4 = a.tris + (size_t)((sc >> 4) + k) * 3;
+            const float4 ta = t4[0], tb = t4[1], tc = t4[2];
+            float t;
+            if (tri_test(ta, tb, tc, r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, t)) {
+                const int tri = __float_as_int(ta.w), surf = __float_as_int(tb.w);
+                if (better(t, surf, tri, h)) {
+                    h.t = t;
+                    h.surf = surf;
+                    h.tri = tri;
+                }
+            }
+        }
+        ++node;
+    }
+    return h;
+}
+
+template <int WG>
+__global__ __launch_bounds__(WG) void plain_kernel(const PlainArgs a) {
+    const int task = (int)(blockIdx.x * WG + threadIdx.x);
+    LaneStats st;
+    Lds lds = {};  // every tally goes to global atomics
+    if (task < a.p.num_rays * a.p.groups) {
+        RayState s;
+        if (start_ray(a.p, a.out, task, s, st)) {
+            Query q;
+            begin_query(q, s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, kBig);
+            ++st.q_b;
+            while (true) {
+                q.h = plain_closest(a, q.r, st.nb, st.tb);
+                HitContext c;
+                if (!close_bounce(a.sv, a.p, a.out, lds, s, c, q, st)) break;
+                unsigned long long m = c.nee;
+                while (m) {  // next-event connections (steam_rt.py:4770-4826)
+                    const int k = __builtin_ctzll(m);
+                    m &= m - 1ull;
+                    const float4 S = a.p.sources[k];
+                    const float tsx = S.x - c.hx, tsy = S.y - c.hy, tsz = S.z - c.hz;
+                    const float D = sqrtf(tsx * tsx + tsy * tsy + tsz * tsz);
+                    if (!(D > a.p.dmin)) continue;
+                    const float sdx = tsx / D, sdy = tsy / D, sdz = tsz / D;
+                    if (!(c.nx * sdx + c.ny * sdy + c.nz * sdz > 0.0f)) continue;
+                    ++st.q_s;
+                    Ray sr;
+                    sr.ox = c.hx;
+                    sr.oy = c.hy;
+                    sr.oz = c.hz;
+                    sr.dx = sdx;
+                    sr.dy = sdy;
+                    sr.dz = sdz;
+                    const Closest sh = plain_closest(a, sr, st.ns, st.ts);
+                    if (sh.surf >= 0 && sh.t > kEps && sh.t < D - kEps) continue;
+                    Query unused;
+                    Connection cn;
+                    if (connect(a.sv, a.p, s, c, k, unused, cn) && cn.bin >= 0)
+                        deposit<false>(a.p, a.out, lds, a.sv.num_surfaces, cn, st);
+                }
+                // the next bounce query (set up and counted by continue_ray)
+                if (continue_ray(a.sv, a.p, a.out, s, c, q, st) == kIdle) break;
+            }
+        }
+    }
+    flush_stats(st, a.p, a.out);
+}
+
 }  // namespace aa
 
 // ---------------------------------------------------------------------------
@@ -154,6 +304,8 @@ void free_scene(aa_scene_s* s) {
     (void)hipSetDevice(s->device);
     (void)hipFree(s->d_nodes);
     (void)hipFree(s->d_tris);
+    (void)hipFree(s->d_plain_nodes);
+    (void)hipFree(s->d_plain_tris);
     (void)hipFree(s->d_normals);
     (void)hipFree(s->d_reflection);
     (void)hipFree(s->d_scattering);
@@ -310,27 +462,51 @@ int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tr
         return fail(AA_ERR_LIMIT, "%zu BVH nodes exceed the 16-bit traversal stack", s->wide.nodes.size());
     }
 
-    // triangles in leaf order: (v0, tri id) (e1, surface) (e2, 0)
-    std::vector<float> tris((size_t)num_triangles * 12);
-    for (int32_t k = 0; k < num_triangles; ++k) {
-        const int64_t tri = bin.prims[(size_t)k];
-        const float* v = tri_vertices + (size_t)tri * 9;
-        float* o = tris.data() + (size_t)k * 12;
-        o[0] = v[0];
-        o[1] = v[1];
-        o[2] = v[2];
-        int32_t id = (int32_t)tri;
-        std::memcpy(&o[3], &id, 4);
-        o[4] = v[3] - v[0];
-        o[5] = v[4] - v[1];
-        o[6] = v[5] - v[2];
-        int32_t sid = tri_surface[tri];
-        std::memcpy(&o[7], &sid, 4);
-        o[8] = v[6] - v[0];
-        o[9] = v[7] - v[1];
-        o[10] = v[8] - v[2];
-        o[11] = 0.0f;
+    // triangles in a tree's primitive order: (v0, tri id) (e1, surface) (e2, 0)
+    auto pack_triangles = [&](const std::vector<int64_t>& prims) {
+        std::vector<float> packed((size_t)num_triangles * 12);
+        for (int32_t k = 0; k < num_triangles; ++k) {
+            const int64_t tri = prims[(size_t)k];
+            const float* v = tri_vertices + (size_t)tri * 9;
+            float* o = packed.data() + (size_t)k * 12;
+            o[0] = v[0];
+            o[1] = v[1];
+            o[2] = v[2];
+            int32_t id = (int32_t)tri;
+            std::memcpy(&o[3], &id, 4);
+            o[4] = v[3] - v[0];
+            o[5] = v[4] - v[1];
+            o[6] = v[5] - v[2];
+            int32_t sid = tri_surface[tri];
+            std::memcpy(&o[7], &sid, 4);
+            o[8] = v[6] - v[0];
+            o[9] = v[7] - v[1];
+            o[10] = v[8] - v[2];
+            o[11] = 0.0f;
+        }
+        return packed;
+    };
+    const std::vector<float> tris = pack_triangles(bin.prims);
+    const std::vector<float> plain_tris = pack_triangles(canonical.prims);
+    // canonical nodes in FP32 (the reference GPU path's AABBs, steam_rt_cuda.py:392-411)
+    const int64_t ncan = canonical.node_count();
+    std::vector<float> plain_nodes((size_t)ncan * 8);
+    for (int64_t n = 0; n < ncan; ++n) {
+        float* o = plain_nodes.data() + (size_t)n * 8;
+        for (int a = 0; a < 3; ++a) {  // rounded outwards, so no box loses a hit
+            const double l = canonical.lo[(size_t)n * 3 + a], u = canonical.hi[(size_t)n * 3 + a];
+            float fl = (float)l, fu = (float)u;
+            if ((double)fl > l) fl = std::nextafter(fl, -INFINITY);
+            if ((double)fu < u) fu = std::nextafter(fu, INFINITY);
+            o[a] = fl;
+            o[4 + a] = fu;
+        }
+        const int32_t esc = (int32_t)canonical.escape[(size_t)n];
+        const int32_t sc = (int32_t)((canonical.start[(size_t)n] << 4) | canonical.count[(size_t)n]);
+        std::memcpy(&o[3], &esc, 4);
+        std::memcpy(&o[7], &sc, 4);
     }
+    s->plain_node_count = (int32_t)ncan;
     std::vector<float> normals((size_t)num_triangles * 4);
     for (int32_t k = 0; k < num_triangles; ++k) {
         normals[(size_t)k * 4 + 0] = tri_normals[(size_t)k * 3 + 0];
@@ -353,7 +529,10 @@ int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tr
     const size_t nrm_bytes = normals.size() * sizeof(float);
     const size_t refl_bytes = (size_t)num_surfaces * 6 * sizeof(float);
     const size_t scat_bytes = (size_t)num_surfaces * sizeof(float);
+    const size_t plain_node_bytes = plain_nodes.size() * sizeof(float);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_nodes, node_bytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_plain_nodes, plain_node_bytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->d_plain_tris, tri_bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_tris, tri_bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_normals, nrm_bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_reflection, refl_bytes);
@@ -361,6 +540,9 @@ int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tr
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_error, sizeof(int));
     if (e == hipSuccess) e = hipMemcpy(s->d_nodes, s->wide.nodes.data(), node_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_tris, tris.data(), tri_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(s->d_plain_nodes, plain_nodes.data(), plain_node_bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(s->d_plain_tris, plain_tris.data(), tri_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_normals, normals.data(), nrm_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_reflection, reflection, refl_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_scattering, scattering, scat_bytes, hipMemcpyHostToDevice);
@@ -369,7 +551,7 @@ int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tr
         free_scene(s);
         return fail(AA_ERR_MEMORY, "scene upload failed: %s", hipGetErrorString(e));
     }
-    s->device_bytes = node_bytes + tri_bytes + nrm_bytes + refl_bytes + scat_bytes + sizeof(int);
+    s->device_bytes = node_bytes + 2 * tri_bytes + plain_node_bytes + nrm_bytes + refl_bytes + scat_bytes + sizeof(int);
     *out = s;
     return AA_OK;
 }
@@ -531,5 +713,31 @@ extern "C" int aa_trace(aa_scene_t scene, const aa_trace_params* p, const float*
     }
 #undef AA_LAUNCH
     if (e != hipSuccess) return fail(AA_ERR_DEVICE, "trace launch failed: %s", hipGetErrorString(e));
+    return AA_OK;
+}
+
+extern "C" int aa_trace_plain(aa_scene_t scene, const aa_trace_params* p, const float* d_sources,
+                              const float* d_directions, const float* d_diffuse_bank, const float* d_diffuse_random,
+                              const int32_t* d_diffuse_indices, const aa_trace_outputs* o, void* stream_ptr) {
+    int rc = aa::validate_trace(scene, p, d_sources, d_directions, d_diffuse_bank, d_diffuse_random,
+                                d_diffuse_indices, o);
+    if (rc != AA_OK) return rc;
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+    AA_HIP(hipSetDevice(scene->device));
+    aa::PlainArgs a;
+    a.sv = aa::make_scene_view(scene, p->use_bvh != 0, 0);
+    a.nodes = reinterpret_cast<const float4*>(scene->d_plain_nodes);
+    a.tris = reinterpret_cast<const float4*>(scene->d_plain_tris);
+    a.num_nodes = scene->plain_node_count;
+    a.p = aa::make_ray_params(p, d_sources, d_directions, d_diffuse_bank, d_diffuse_random, d_diffuse_indices);
+    a.out = aa::make_outputs(o, p);
+    const aa::OutputExtents ext = aa::output_extents(o, p, scene->num_surfaces);
+    for (int i = 0; i < 8; ++i) AA_HIP(zero_range(ext.ptr[i], ext.bytes[i], stream));
+    constexpr int kPlainBlock = 128;  // the reference's block size (steam_rt_cuda.py:136-137)
+    const int64_t tasks = (int64_t)p->num_rays * a.p.groups;
+    const int grid = (int)((tasks + kPlainBlock - 1) / kPlainBlock);
+    hipLaunchKernelGGL(aa::plain_kernel<kPlainBlock>, dim3(grid), dim3(kPlainBlock), 0, stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(AA_ERR_DEVICE, "plain trace launch failed: %s", hipGetErrorString(e));
     return AA_OK;
 }

@@ -320,8 +320,13 @@ def make_params(*, num_rays: int, num_bounces: int, num_bins: int, num_diffuse_s
 
 
 def launch_trace(scene: DeviceScene, params: _lib.TraceParams, sources_dev, bank_dev, buffers: TraceBuffers, *,
-                 directions_dev=None, random_dev=None, indices_dev=None, stream=None) -> None:
-    """Enqueue one trace on ``stream`` (torch stream or None for the current one)."""
+                 directions_dev=None, random_dev=None, indices_dev=None, stream=None, plain: bool = False) -> None:
+    """Enqueue one trace on ``stream`` (torch stream or None for the current one).
+
+    ``plain`` runs the one-thread-per-ray kernel of the reference GPU
+    backend's execution model (``aa_trace_plain``) instead of the persistent
+    tracer; the results are the same.
+    """
     torch = _torch()
     lib = _lib.load()
     s = stream if stream is not None else torch.cuda.current_stream(scene.device)
@@ -330,8 +335,9 @@ def launch_trace(scene: DeviceScene, params: _lib.TraceParams, sources_dev, bank
     def p(t):
         return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
-    _lib.check(lib.aa_trace(scene.handle, ctypes.byref(params), p(sources_dev), p(directions_dev), p(bank_dev),
-                            p(random_dev), p(indices_dev), ctypes.byref(out), ctypes.c_void_p(s.cuda_stream)))
+    fn = lib.aa_trace_plain if plain else lib.aa_trace
+    _lib.check(fn(scene.handle, ctypes.byref(params), p(sources_dev), p(directions_dev), p(bank_dev),
+                  p(random_dev), p(indices_dev), ctypes.byref(out), ctypes.c_void_p(s.cuda_stream)))
 
 
 def trace_energy_field_hip(
@@ -371,6 +377,7 @@ def trace_energy_field_hip(
     log_rays: int = 0,
     log_bounces: int = 0,
     tuning: Mapping[str, int] | None = None,
+    plain: bool = False,
 ) -> dict[str, Any]:
     if not hip_available(device_id):
         raise RuntimeError(f"HIP device {device_id} is not available")
@@ -438,7 +445,7 @@ def trace_energy_field_hip(
     end_ev = torch.cuda.Event(enable_timing=True)
     start_ev.record(stream)
     launch_trace(scene, params, sources_dev, bank_dev, buffers, directions_dev=dirs_dev, random_dev=rand_dev,
-                 indices_dev=idx_dev, stream=stream)
+                 indices_dev=idx_dev, stream=stream, plain=plain)
     end_ev.record(stream)
     end_ev.synchronize()
     kernel_time = start_ev.elapsed_time(end_ev) / 1000.0

@@ -14,6 +14,7 @@ band RT60 / EDT error of the GPU result.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import statistics
@@ -62,8 +63,12 @@ def band_errors(a: np.ndarray, b: np.ndarray, bin_dur: float) -> dict[str, float
     return out
 
 
+def _progress(line: str) -> None:
+    print(line, file=sys.stderr, flush=True)
+
+
 def _batch_vs_plain(requests, label: str, steps: int, warmup: int, plain_sample: int, use_graph: bool,
-                    tune: bool) -> dict:
+                    tune: bool, tune_sample: int) -> dict:
     """Time the persistent batch (autotuned) against plain one-trace-per-launch."""
     import torch
 
@@ -73,26 +78,40 @@ def _batch_vs_plain(requests, label: str, steps: int, warmup: int, plain_sample:
     engine = BatchEngine(0, max_traces=len(requests))
     t0 = time.perf_counter()
     tuner = Autotuner(0)
-    tuning = tuner.tune_batch(requests, sample=min(64, len(requests)), reps=1) if tune else BatchTuning()
+    tuning = tuner.tune_batch(requests, sample=min(tune_sample, len(requests)), reps=1,
+                              progress=_progress) if tune else BatchTuning()
     tune_s = time.perf_counter() - t0
     tuning.use_graph = use_graph
     for _ in range(warmup):
         engine.submit(requests, tuning).wait()
     torch.cuda.synchronize()
+    _progress(f"batch warm-up done ({warmup} submissions)")
     times = []
-    for _ in range(steps):
+    for i in range(steps):
         t = time.perf_counter()
         res = engine.submit(requests, tuning).wait()
         times.append(time.perf_counter() - t)
+        _progress(f"batch step {i}: {times[-1]:.3f} s")
     rirs = sum(r.num_sources for r in requests)
     rays = sum(r.num_rays for r in requests)
     queries = sum(x["counters"]["bounce_queries"] + x["counters"]["shadow_queries"] for x in res)
     med = float(np.median(times))
+    # one-scene-per-launch baselines on a sample of single-source traces (a
+    # keyframe trace of C3 is one source): the plain kernel of the reference
+    # GPU backend's execution model, and this build's persistent single-trace
+    # kernel
+    units = [dataclasses.replace(r, sources=np.asarray(r.sources).reshape(-1, 3)[k:k + 1], tag=(r.tag, k))
+             for r in requests for k in range(r.num_sources)]
     rng = np.random.default_rng(0)
-    pick = sorted(rng.choice(len(requests), size=min(plain_sample, len(requests)), replace=False))
-    sample = [requests[int(i)] for i in pick]
-    _, plain_s = trace_sequential(sample)
-    plain_rirs_per_s = sum(r.num_sources for r in sample) / plain_s
+    pick = sorted(rng.choice(len(units), size=min(plain_sample, len(units)), replace=False))
+    sample = [units[int(i)] for i in pick]
+    trace_sequential(sample[:1], plain=True)
+    _, plain_s = trace_sequential(sample, plain=True)
+    _progress(f"plain kernel: {len(sample)} traces in {plain_s:.3f} s")
+    _, single_s = trace_sequential(sample)
+    _progress(f"persistent single-trace kernel: {len(sample)} traces in {single_s:.3f} s")
+    plain_rirs_per_s = len(sample) / plain_s
+    single_rirs_per_s = len(sample) / single_s
     engine.close()
     return {
         "workload": label,
@@ -107,6 +126,8 @@ def _batch_vs_plain(requests, label: str, steps: int, warmup: int, plain_sample:
         "plain_sample_traces": len(sample),
         "plain_rirs_per_s": round(plain_rirs_per_s, 3),
         "speedup_vs_plain": round((rirs / med) / plain_rirs_per_s, 2),
+        "single_trace_rirs_per_s": round(single_rirs_per_s, 3),
+        "speedup_vs_single_trace": round((rirs / med) / single_rirs_per_s, 2),
     }
 
 
@@ -123,10 +144,12 @@ def run_batched(args) -> int:
         label = f"{args.scenes or 32} furnished L plans x 64-keyframe moving source (one multi-source trace per " \
                 f"scene), 100k rays, {args.bounces} bounces, 1.0 s @ 1 ms, hipGraph replay"
     build_s = time.perf_counter() - t
+    _progress(f"{len(reqs)} requests built in {build_s:.1f} s")
+    # the tuning sample is a few traces' worth of lane tasks (a C3 trace carries 64 sources)
     out = _batch_vs_plain(reqs, label, args.steps, args.warmup, args.plain_sample, args.workload == "c3",
-                          not args.no_tune)
+                          not args.no_tune, tune_sample=64 if args.workload == "c2" else 4)
     out["scene_build_s"] = round(build_s, 2)
-    print(json.dumps({"metric": "RIRs/s (batched persistent vs plain one-trace-per-launch)", "unit": "RIRs/s",
+    print(json.dumps({"metric": "RIRs/s (batched persistent vs plain one-scene-per-launch)", "unit": "RIRs/s",
                       "value": out["rirs_per_s"], "dtype": "f32", "data": "synthetic", "config": out}), flush=True)
     return 0
 
@@ -170,6 +193,9 @@ def main() -> int:
     ap.add_argument("--triangles", type=int, default=20_000)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plain", action="store_true",
+                    help="c1: time the plain one-thread-per-ray kernel (the reference GPU backend's execution "
+                         "model) instead of the persistent tracer")
     ap.add_argument("--cpu-budget-s", type=float, default=25.0, help="bound on the CPU baseline's wall time")
     ap.add_argument("--tuning", type=str, default="", help="JSON dict of launch overrides")
     ap.add_argument("--copy-probe", action="store_true",
@@ -221,7 +247,7 @@ def main() -> int:
     stream = torch.cuda.current_stream(0)
 
     def step():
-        hip_backend.launch_trace(scene, params, src, bank, buffers, stream=stream)
+        hip_backend.launch_trace(scene, params, src, bank, buffers, stream=stream, plain=args.plain)
 
     for _ in range(args.warmup):
         step()
@@ -263,6 +289,7 @@ def main() -> int:
             "bounces": B,
             "bins": K,
             "rng": "counter (Philox4x32-10)",
+            "kernel": "plain one-thread-per-ray (reference execution model)" if args.plain else "persistent",
             "tuning": tuning or "default",
         },
         "primary_rays_per_s": round(N / (ms_per_step / 1000.0), 1),

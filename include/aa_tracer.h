@@ -159,6 +159,18 @@ int aa_trace(aa_scene_t scene, const aa_trace_params *params, const float *d_sou
              const float *d_diffuse_bank, const float *d_diffuse_random, const int32_t *d_diffuse_indices,
              const aa_trace_outputs *outputs, void *stream);
 
+/* The plain one-scene-per-launch tracer (replaces the execution model of
+ * the reference GPU backend, _trace_energy_cuda_kernel steam_rt_cuda.py:
+ * 491-710, launched by trace_energy_field_cuda steam_rt_cuda.py:136-160):
+ * one thread per ray, blocks of 128, no persistence or refill, a stackless
+ * walk of the canonical binary BVH, shadow queries answered by a full
+ * closest-hit walk.  Same arguments and results as aa_trace (the bounce
+ * arithmetic is shared); kept as the baseline the persistent and batched
+ * paths are measured against.  Tuning fields of `params` are ignored. */
+int aa_trace_plain(aa_scene_t scene, const aa_trace_params *params, const float *d_sources,
+                   const float *d_directions, const float *d_diffuse_bank, const float *d_diffuse_random,
+                   const int32_t *d_diffuse_indices, const aa_trace_outputs *outputs, void *stream);
+
 /* Synchronise `stream` and report a kernel-side overflow flag, if any. */
 int aa_stream_sync(void *stream);
 

@@ -47,7 +47,8 @@ class Case:
         return float(np.linalg.norm(np.asarray(s) - self.workload.listener)) / C
 
 
-def run_gpu(case: Case, *, use_bvh: bool = True, tuning=None, log_rays: int = 0, scene=None, source=None):
+def run_gpu(case: Case, *, use_bvh: bool = True, tuning=None, log_rays: int = 0, scene=None, source=None,
+            plain: bool = False):
     from acoustic_agent_amd import hip_backend
 
     dirs, bank, U, J = case.ray_set()
@@ -64,7 +65,7 @@ def run_gpu(case: Case, *, use_bvh: bool = True, tuning=None, log_rays: int = 0,
         source_forward_vector=np.asarray((1.0, 0.0, 0.0)), dipole_weight=0.0, dipole_power=1.0,
         render_ambisonics=case.ambisonics, device_id=0, rng_mode=case.rng, seed=case.seed,
         num_rays=case.num_rays, scene=scene, tessellation_max_edge=w.max_edge, log_rays=log_rays,
-        log_bounces=case.log_bounces if log_rays else 0, tuning=tuning, **kw)
+        log_bounces=case.log_bounces if log_rays else 0, tuning=tuning, plain=plain, **kw)
 
 
 def run_oracle(case: Case, precision: str = "float32", *, triangles: bool = True, log_rays: int = 0,

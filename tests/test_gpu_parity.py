@@ -137,3 +137,44 @@ def test_ambisonic_channel_ratio_is_initial_direction(shoebox):
     assert np.count_nonzero(valid) > 1
     for ch in range(3):
         np.testing.assert_allclose(gpu["ambisonic"][0, ch + 1, valid] / w[valid], d[ch], rtol=1e-6, atol=1e-7)
+
+
+@pytest.mark.parametrize("rng", ["reference", "counter"])
+def test_plain_kernel_matches_fp32_oracle_ray_for_ray(shoebox, rng):
+    # the one-thread-per-ray baseline kernel gives the same trajectories
+    case = Case(shoebox, num_rays=4096, num_bounces=50, num_bins=1000, rng=rng, seed=5, log_bounces=50)
+    gpu = run_gpu(case, log_rays=4096, plain=True)
+    ref = run_oracle(case, "float32", log_rays=4096)
+    assert np.array_equal(gpu["log_triangle"], ref["log_prim"])
+    assert np.array_equal(gpu["hit_counts"], ref["hit_counts"])
+    assert np.array_equal(gpu["contrib_counts"], ref["contrib_counts"])
+    assert gpu["active_count"] == ref["active_count"]
+    assert gpu["actual_bounces"] == ref["actual_bounces"]
+    assert rel_l1(gpu["echogram"], ref["echogram"]) <= 1e-5
+    assert gpu["counters"]["bounce_queries"] == ref["counters"]["bounce_queries"]
+    assert gpu["counters"]["shadow_queries"] == ref["counters"]["shadow_queries"]
+
+
+@pytest.mark.parametrize("use_bvh", [True, False])
+def test_plain_kernel_equals_persistent_kernel(lplan, use_bvh):
+    n = 4096 if use_bvh else 256
+    case = Case(lplan, num_rays=n, num_bounces=60, num_bins=2000, seed=11, rng="counter", log_bounces=60)
+    a = run_gpu(case, use_bvh=use_bvh, log_rays=n)
+    b = run_gpu(case, use_bvh=use_bvh, log_rays=n, plain=True)
+    assert np.array_equal(a["log_triangle"], b["log_triangle"])
+    assert np.array_equal(a["hit_counts"], b["hit_counts"])
+    assert np.array_equal(a["contrib_counts"], b["contrib_counts"])
+    assert a["counters"]["shadow_queries"] == b["counters"]["shadow_queries"]
+    assert rel_l1(b["echogram"], a["echogram"]) <= 1e-5
+
+
+def test_plain_kernel_multi_source(lplan):
+    rng = np.random.default_rng(12)
+    srcs = np.stack([np.asarray(scenes.L_SOURCE) + rng.uniform(-0.5, 0.5, 3) * (1, 1, 0.2) for _ in range(3)])
+    case = Case(lplan, num_rays=2048, num_bounces=60, num_bins=1000, seed=13, rng="counter", sources=srcs)
+    a = run_gpu(case)
+    b = run_gpu(case, plain=True)
+    assert np.array_equal(a["hit_counts"], b["hit_counts"])
+    assert np.array_equal(a["active_count"], b["active_count"])
+    assert np.array_equal(a["actual_bounces"], b["actual_bounces"])
+    assert rel_l1(b["echogram"], a["echogram"]) <= 1e-5
