@@ -405,31 +405,45 @@ __device__ __forceinline__ void begin_query(Query& q, float ox, float oy, float 
     q.occluded = false;
 }
 
-// Triangles [start, start + count) against the query: bounce queries keep
-// the closest hit, shadow queries stop at the first hit before the source
-// (tri_test already requires t > EPS, the lower end of (EPS, tmax - EPS)).
-// One copy serves both kinds, so lanes of either kind test together.
+// One triangle against the query: bounce queries keep the closest hit,
+// shadow queries flag a hit before the source (tri_test already requires
+// t > EPS, the lower end of (EPS, tmax - EPS)).
+__device__ __forceinline__ void test_one(const float4 ta, const float4 tb, const float4 tc, Query& q, bool shadow) {
+    float t;
+    if (tri_test(ta, tb, tc, q.r.ox, q.r.oy, q.r.oz, q.r.dx, q.r.dy, q.r.dz, t)) {
+        if (shadow) {
+            if (t < q.tmax - kEps) q.occluded = true;
+        } else {
+            const int tri = __float_as_int(ta.w), surf = __float_as_int(tb.w);
+            if (better(t, surf, tri, q.h)) {
+                q.h.t = t;
+                q.h.surf = surf;
+                q.h.tri = tri;
+            }
+        }
+    }
+}
+
+// Triangles [start, start + count) against the query.  Triangles are loaded
+// two at a time (the second index clamped into the range, so both loads are
+// issued before the first test waits): a leaf costs one dependent round trip
+// to L2 per pair instead of one per triangle.  One copy serves both query
+// kinds, so lanes of either kind test together.
 __device__ __forceinline__ void test_leaf(const SceneView& sv, int start, int count, Query& q, bool shadow,
                                           unsigned& tri_cnt) {
-    for (int k = 0; k < count; ++k) {
+    for (int k = 0; k < count; k += 2) {
+        const bool two = k + 1 < count;
+        const float4* t0 = sv.tris + (size_t)(start + k) * 3;
+        const float4* t1 = sv.tris + (size_t)(start + (two ? k + 1 : k)) * 3;
+        const float4 a0 = t0[0], b0 = t0[1], c0 = t0[2];
+        const float4 a1 = t1[0], b1 = t1[1], c1 = t1[2];
         ++tri_cnt;
-        const float4* t4 = sv.tris + (size_t)(start + k) * 3;
-        const float4 ta = t4[0], tb = t4[1], tc = t4[2];
-        float t;
-        if (tri_test(ta, tb, tc, q.r.ox, q.r.oy, q.r.oz, q.r.dx, q.r.dy, q.r.dz, t)) {
-            if (shadow) {
-                if (t < q.tmax - kEps) {
-                    q.occluded = true;
-                    break;
-                }
-            } else {
-                const int tri = __float_as_int(ta.w), surf = __float_as_int(tb.w);
-                if (better(t, surf, tri, q.h)) {
-                    q.h.t = t;
-                    q.h.surf = surf;
-                    q.h.tri = tri;
-                }
-            }
+        test_one(a0, b0, c0, q, shadow);
+        if (q.occluded) break;
+        if (two) {
+            ++tri_cnt;
+            test_one(a1, b1, c1, q, shadow);
+            if (q.occluded) break;
         }
     }
 }
