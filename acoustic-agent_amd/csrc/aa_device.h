@@ -124,6 +124,30 @@ __device__ __forceinline__ void philox4x32_10(unsigned c0, unsigned c1, unsigned
     r1 = c1;
 }
 
+// Address-space-typed memory operations.  The kernel argument structs hold
+// generic pointers, for which the compiler emits FLAT instructions (they
+// count against both the vector-memory and the LDS wait counters and check
+// the aperture per lane); these casts make it emit global_* and ds_*.
+typedef __attribute__((address_space(1))) float global_float;
+typedef __attribute__((address_space(1))) int global_int;
+typedef __attribute__((address_space(1))) float global_cfloat;
+typedef __attribute__((address_space(3))) float shared_float;
+typedef __attribute__((address_space(3))) float shared_cfloat;
+typedef __attribute__((address_space(3))) int shared_int;
+
+__device__ __forceinline__ void add_global(float* p, float v) {
+    __hip_atomic_fetch_add((global_float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void add_global(int* p, int v) {
+    __hip_atomic_fetch_add((global_int*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void add_shared(float* p, float v) {
+    __hip_atomic_fetch_add((shared_float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void add_shared(int* p, int v) {
+    __hip_atomic_fetch_add((shared_int*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ float safe_inv(float d) {
     // Axis with |d| <= 1e-12: the slab test must only require the origin to
     // lie inside the slab (steam_rt_cuda.py:416-419); a huge signed inverse
@@ -178,14 +202,18 @@ __device__ __forceinline__ bool better(float t, int surf, int tri, const Closest
     return false;
 }
 
+// A 4-wide node from the LDS-staged top levels or from global memory (typed
+// pointers in both branches keep the compiler from merging them into FLAT
+// loads).
 __device__ __forceinline__ void load_node(const SceneView& sv, const float4* s_nodes, int node, float4* v) {
     if (node < sv.lds_nodes) {
+        const shared_cfloat* p = (const shared_cfloat*)(s_nodes + node * 8);
 #pragma unroll
-        for (int r = 0; r < 7; ++r) v[r] = s_nodes[node * 8 + r];
+        for (int r = 0; r < 7; ++r) v[r] = make_float4(p[4 * r], p[4 * r + 1], p[4 * r + 2], p[4 * r + 3]);
     } else {
-        const float4* g = sv.nodes + (size_t)node * 8;
+        const global_cfloat* g = (const global_cfloat*)(sv.nodes + (size_t)node * 8);
 #pragma unroll
-        for (int r = 0; r < 7; ++r) v[r] = g[r];
+        for (int r = 0; r < 7; ++r) v[r] = make_float4(g[4 * r], g[4 * r + 1], g[4 * r + 2], g[4 * r + 3]);
     }
 }
 
@@ -615,25 +643,25 @@ __device__ __forceinline__ void deposit(const RayParams& p, const Outputs& out, 
         const float e = cn.e[b];
         const size_t hi = ((size_t)k * kBands + b) * K + cn.bin;
         if (LDS_HIST)
-            atomicAdd(&lds.hist[hi], e);
+            add_shared(&lds.hist[hi], e);
         else
-            atomicAdd(&out.echogram[hi], e);
+            add_global(&out.echogram[hi], e);
         if (p.amb) {
             float* am = out.ambisonic + (((size_t)k * kBands + b) * 4) * K + cn.bin;
-            atomicAdd(am, e);
-            atomicAdd(am + K, e * cn.d0x);
-            atomicAdd(am + 2 * K, e * cn.d0y);
-            atomicAdd(am + 3 * K, e * cn.d0z);
+            add_global(am, e);
+            add_global(am + K, e * cn.d0x);
+            add_global(am + 2 * K, e * cn.d0y);
+            add_global(am + 3 * K, e * cn.d0z);
         }
         esum += e;
     }
     ++st.contrib;
     if (lds.hits) {
-        atomicAdd(&lds.contrib[surf], 1);
-        atomicAdd(&lds.senergy[surf], esum);
+        add_shared(&lds.contrib[surf], 1);
+        add_shared(&lds.senergy[surf], esum);
     } else {
-        atomicAdd(&out.contrib_counts[(size_t)k * num_surfaces + surf], 1);
-        atomicAdd(&out.surface_energy[(size_t)k * num_surfaces + surf], esum);
+        add_global(&out.contrib_counts[(size_t)k * num_surfaces + surf], 1);
+        add_global(&out.surface_energy[(size_t)k * num_surfaces + surf], esum);
     }
 }
 
@@ -801,9 +829,9 @@ __device__ __forceinline__ bool close_bounce(const SceneView& sv, const RayParam
         const int k = __builtin_ctzll(m);
         m &= m - 1ull;
         if (lds.hits)
-            atomicAdd(&lds.hits[c.surf], 1);
+            add_shared(&lds.hits[c.surf], 1);
         else
-            atomicAdd(&out.hit_counts[(size_t)k * sv.num_surfaces + c.surf], 1);
+            add_global(&out.hit_counts[(size_t)k * sv.num_surfaces + c.surf], 1);
     }
     c.nee = s.mask;
     return true;
