@@ -164,9 +164,11 @@ __device__ __forceinline__ bool tri_test(const float4 a, const float4 b, const f
     const float py = dz * e2x - dx * e2z;
     const float pz = dx * e2y - dy * e2x;
     const float det = e1x * px + e1y * py + e1z * pz;
-    if (!(fabsf(det) > 1e-12f)) return false;
-    const float inv = 1.0f / det;
     const float sx = ox - a.x, sy = oy - a.y, sz = oz - a.z;
+    // the (never true) NaN test ties a's load to the first branch, so it is
+    // issued with the others instead of after the determinant test
+    if (!(fabsf(det) > 1e-12f) || (sx != sx)) return false;
+    const float inv = 1.0f / det;
     const float u = (sx * px + sy * py + sz * pz) * inv;
     if (u < -1e-6f || u > 1.000001f) return false;
     const float qx = sy * e1z - sz * e1y;
