@@ -134,12 +134,13 @@ def _plan(requests: Sequence[TraceRequest]) -> _Layout:
 class BatchResult:
     """Device outputs of one submission; ``wait`` copies them back."""
 
-    def __init__(self, requests, layout, arenas, event, device) -> None:
+    def __init__(self, requests, layout, arenas, event, device, host=None) -> None:
         self.requests = requests
         self.layout = layout
         self.arenas = arenas
         self.event = event
         self.device = device
+        self.host_arenas = host  # pinned copies queued on the submission stream (or None)
         self._host = None
 
     def done(self) -> bool:
@@ -148,14 +149,17 @@ class BatchResult:
     def wait(self) -> list[dict[str, Any]]:
         if self._host is None:
             self.event.synchronize()
-            f32, i32, i64 = (a.cpu().numpy() for a in self.arenas[:3])
+            if self.host_arenas is not None:
+                f32, i32, i64 = (a.numpy() for a in self.host_arenas)
+            else:
+                f32, i32, i64 = (a.cpu().numpy() for a in self.arenas[:3])
             out = []
             for r, slot in zip(self.requests, self.layout.slots):
                 Ks, K, S = r.num_sources, r.num_bins, r.scene.num_surfaces
 
                 def take(arr, name, shape):
                     off, n = slot[name]
-                    return arr[off:off + n].reshape(shape)
+                    return arr[off:off + n].reshape(shape).copy()
 
                 res = {
                     "echogram": take(f32, "echogram", (Ks, 6, K)),
@@ -206,7 +210,13 @@ class BatchEngine:
             pass
 
     def submit(self, requests: Sequence[TraceRequest], tuning: BatchTuning | None = None, stream=None,
-               order: bool = True) -> BatchResult:
+               order: bool = True, copy_back: bool = False) -> BatchResult:
+        """Enqueue one persistent batch on ``stream``.
+
+        ``copy_back`` also queues the copies of every output into pinned host
+        memory on the same stream, so that ``done()`` means the results are
+        on the host and reading them waits on nothing else.
+        """
         torch = self.torch
         if not requests:
             raise ValueError("empty batch")
@@ -243,9 +253,15 @@ class BatchEngine:
         ts = tuning.as_struct()
         _lib.check(_lib.load().aa_batch_submit(self._handle, table, len(reqs), ctypes.byref(ts),
                                                ctypes.c_void_p(s.cuda_stream)))
+        host = None
+        if copy_back:
+            host = tuple(torch.empty(a.shape, dtype=a.dtype, pin_memory=True) for a in (f32, i32, i64))
+            with torch.cuda.stream(s):
+                for h, a in zip(host, (f32, i32, i64)):
+                    h.copy_(a, non_blocking=True)
         event = torch.cuda.Event()
         event.record(s)
-        return BatchResult(reqs, lay, (f32, i32, i64, src), event, self.device)
+        return BatchResult(reqs, lay, (f32, i32, i64, src), event, self.device, host)
 
 
 def trace_sequential(requests: Sequence[TraceRequest], stream=None,

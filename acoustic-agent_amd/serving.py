@@ -58,12 +58,14 @@ def summarize(completions: list[Completion], makespan_s: float) -> dict[str, Any
     for c in completions:
         classes.setdefault(c.request.size_class, []).append(c.latency_s)
     traces = sum(len(c.request.traces) for c in completions)
+    rirs = sum(t.num_sources for c in completions for t in c.request.traces)
     rays = sum(t.num_rays for c in completions for t in c.request.traces)
     return {
         "requests": len(completions),
         "traces": traces,
+        "rirs": rirs,
         "makespan_s": round(makespan_s, 4),
-        "rirs_per_s": round(traces / makespan_s, 3) if makespan_s > 0 else None,
+        "rirs_per_s": round(rirs / makespan_s, 3) if makespan_s > 0 else None,
         "primary_rays_per_s": round(rays / makespan_s, 1) if makespan_s > 0 else None,
         "latency_all": _percentiles([c.latency_s for c in completions]),
         "latency_by_class": {k: _percentiles(v) for k, v in sorted(classes.items())},
@@ -110,28 +112,16 @@ class ServingLoop:
             while i < len(reqs) and (offset is None or reqs[i].arrival_s <= now):
                 pending.append(reqs[i])
                 i += 1
-            # retire finished batches
-            still = []
-            for slot, res, members, nbytes in inflight:
-                if res.done():
-                    results = res.wait()
-                    t_done = time.perf_counter() - t0
-                    by_req: dict[int, list] = {}
-                    for r in results:
-                        by_req.setdefault(r["tag"][0], []).append(r)
-                    for k, m in enumerate(members):
-                        rs = sorted(by_req.get(k, []), key=lambda r: r["tag"][1])
-                        comp = Completion(m, t_done - (m.arrival_s if offset is not None else 0.0), rs)
-                        if self.compute_rt60 and rs:
-                            comp.rt60 = _rt60(rs[0]["echogram"], m.traces[0].bin_dur)
-                        done.append(comp)
+            # batches whose results (copied back on their own stream) are on the host
+            finished = [x for x in inflight if x[1].done()]
+            if finished:
+                inflight = [x for x in inflight if all(x is not f for f in finished)]
+                for slot, _, _, nbytes in finished:
                     free.append(slot)
                     in_use -= nbytes
-                else:
-                    still.append((slot, res, members, nbytes))
-            inflight = still
-            # launch a new batch on a free slot
-            if free and pending:
+            # refill free slots before any host-side work
+            launched = False
+            while free and pending:
                 members, traces, nbytes = [], [], 0
                 while pending:
                     r = pending[0]
@@ -144,13 +134,25 @@ class ServingLoop:
                         traces.append(t)
                     members.append(r)
                     nbytes += b
-                if members:
-                    slot = free.pop()
-                    res = self.engines[slot].submit(traces, self.tuning, stream=self.streams[slot])
-                    inflight.append((slot, res, members, nbytes))
-                    in_use += nbytes
-                    continue
-            if not (free and pending):
+                if not members:
+                    break
+                slot = free.pop()
+                res = self.engines[slot].submit(traces, self.tuning, stream=self.streams[slot], copy_back=True)
+                inflight.append((slot, res, members, nbytes))
+                in_use += nbytes
+                launched = True
+            # host-side work of the finished batches: split per request, RT60
+            for slot, res, members, nbytes in finished:
+                results = res.wait()
+                by_req: dict[int, list] = {}
+                for r in results:
+                    by_req.setdefault(r["tag"][0], []).append(r)
+                for k, m in enumerate(members):
+                    rs = sorted(by_req.get(k, []), key=lambda r: r["tag"][1])
+                    rt60 = _rt60(rs[0]["echogram"], m.traces[0].bin_dur) if self.compute_rt60 and rs else None
+                    t_done = time.perf_counter() - t0  # histogram and RT60 ready
+                    done.append(Completion(m, t_done - (m.arrival_s if offset is not None else 0.0), rs, rt60))
+            if not finished and not launched:
                 time.sleep(0.0002)
         makespan = time.perf_counter() - t0
         return done, summarize(done, makespan)
@@ -167,6 +169,7 @@ def serve_sequential(stream: Sequence[ServedRequest], realtime: bool = True) -> 
             if wait > 0:
                 time.sleep(wait)
         results, _ = trace_sequential(r.traces)
-        t_done = time.perf_counter() - t0
-        done.append(Completion(r, t_done - (r.arrival_s if realtime else 0.0), results))
+        rt60 = _rt60(results[0]["echogram"], r.traces[0].bin_dur) if results else None
+        t_done = time.perf_counter() - t0  # histogram and RT60 ready, as for the served path
+        done.append(Completion(r, t_done - (r.arrival_s if realtime else 0.0), results, rt60))
     return done, summarize(done, time.perf_counter() - t0)
