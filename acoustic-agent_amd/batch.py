@@ -294,4 +294,6 @@ def trace_sequential(requests: Sequence[TraceRequest], stream=None,
             "counters": dict(zip(_lib.COUNTER_NAMES, (int(x) for x in buf.counters.cpu().numpy()))),
             "tag": r.tag,
         })
+        if r.render_ambisonics:
+            out[-1]["ambisonic"] = buf.ambisonic.cpu().numpy()
     return out, time.perf_counter() - t0

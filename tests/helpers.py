@@ -62,7 +62,9 @@ def run_gpu(case: Case, *, use_bvh: bool = True, tuning=None, log_rays: int = 0,
         arrays=case.arrays, use_bvh=use_bvh, num_bounces=case.num_bounces, num_bins=case.num_bins,
         bin_dur=case.bin_dur, speed_of_sound=C, max_path_len=case.duration * C, listener_radius=0.1,
         source_radius=0.1, irradiance_min_distance=1.0, specular_exponent=100.0,
-        source_forward_vector=np.asarray((1.0, 0.0, 0.0)), dipole_weight=0.0, dipole_power=1.0,
+        source_forward_vector=np.asarray(case.extra.get("source_forward", (1.0, 0.0, 0.0))),
+        dipole_weight=float(case.extra.get("dipole_weight", 0.0)),
+        dipole_power=float(case.extra.get("dipole_power", 1.0)),
         render_ambisonics=case.ambisonics, device_id=0, rng_mode=case.rng, seed=case.seed,
         num_rays=case.num_rays, scene=scene, tessellation_max_edge=w.max_edge, log_rays=log_rays,
         log_bounces=case.log_bounces if log_rays else 0, tuning=tuning, plain=plain, **kw)
@@ -77,6 +79,9 @@ def run_oracle(case: Case, precision: str = "float32", *, triangles: bool = True
     return O.trace(scene, source=src, listener=w.listener, num_rays=case.num_rays, num_bounces=case.num_bounces,
                    num_bins=case.num_bins, bin_dur=case.bin_dur, c=C, max_path_len=case.duration * C,
                    direct_delay=case.direct_delay(src), diffuse_bank=bank, rng=case.rng, directions=dirs,
+                   source_forward=case.extra.get("source_forward", (1.0, 0.0, 0.0)),
+                   dipole_weight=float(case.extra.get("dipole_weight", 0.0)),
+                   dipole_power=float(case.extra.get("dipole_power", 1.0)),
                    diffuse_random=U, diffuse_indices=J, seed=case.seed, render_ambisonics=case.ambisonics,
                    log_rays=log_rays, log_bounces=case.log_bounces if log_rays else 0, threads=threads,
                    ray_range=ray_range)

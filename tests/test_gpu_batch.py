@@ -100,3 +100,21 @@ def test_batch_rejects_oversize_and_empty(requests):
     with pytest.raises(ValueError):
         engine.submit(requests[:5])
     engine.close()
+
+
+def test_batch_ambisonics_equal_single_traces():
+    # FOA echograms through the batch path (per-trace render_ambisonics)
+    box = scenes.shoebox_workload(1)
+    s_box = _scene(box)
+    reqs = [TraceRequest(scene=s_box, listener=box.listener, sources=box.source[None, :], num_rays=3000,
+                         num_bounces=40, duration_s=1.0, seed=77 + i, render_ambisonics=bool(i % 2), tag=i)
+            for i in range(4)]
+    single = {r["tag"]: r for r in trace_sequential(reqs)[0]}
+    engine = BatchEngine(0, max_traces=8)
+    got = {r["tag"]: r for r in engine.submit(reqs).wait()}
+    engine.close()
+    for i in range(4):
+        _check_same(got[i], single[i])
+        if i % 2:
+            assert got[i]["ambisonic"].shape == (1, 6, 4, 1000)
+            assert rel_l1(got[i]["ambisonic"], single[i]["ambisonic"]) <= 1e-5
