@@ -158,8 +158,14 @@ class ServingLoop:
         return done, summarize(done, makespan)
 
 
-def serve_sequential(stream: Sequence[ServedRequest], realtime: bool = True) -> tuple[list[Completion], dict[str, Any]]:
-    """Baseline: one request at a time, one plain launch per trace."""
+def serve_sequential(stream: Sequence[ServedRequest], realtime: bool = True,
+                     plain: bool = False) -> tuple[list[Completion], dict[str, Any]]:
+    """Baseline: one request at a time, one launch per trace.
+
+    ``plain`` runs each trace on the plain one-thread-per-ray kernel (the
+    reference GPU backend's execution model); otherwise on the persistent
+    single-trace kernel.
+    """
     reqs = sorted(stream, key=lambda r: r.arrival_s)
     done = []
     t0 = time.perf_counter()
@@ -168,7 +174,7 @@ def serve_sequential(stream: Sequence[ServedRequest], realtime: bool = True) -> 
             wait = r.arrival_s - (time.perf_counter() - t0)
             if wait > 0:
                 time.sleep(wait)
-        results, _ = trace_sequential(r.traces)
+        results, _ = trace_sequential(r.traces, plain=plain)
         rt60 = _rt60(results[0]["echogram"], r.traces[0].bin_dur) if results else None
         t_done = time.perf_counter() - t0  # histogram and RT60 ready, as for the served path
         done.append(Completion(r, t_done - (r.arrival_s if realtime else 0.0), results, rt60))
