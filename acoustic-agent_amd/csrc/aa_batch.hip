@@ -69,6 +69,22 @@ __global__ void batch_zero_kernel(const BatchHeader* hdr, const BatchTrace* trac
     }
 }
 
+// A wave-uniform copy of *src, word by word through readfirstlane.  The batch
+// table lives in ordinary global memory that the kernel also writes (through
+// other pointers), so the compiler cannot prove it unchanged and would read
+// each field with vector loads, again at every use; the copy puts the
+// fields in scalar registers once per work item.
+template <typename T>
+__device__ __forceinline__ T uniform_copy(const T* src) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized struct");
+    T dst;
+    const int* s = reinterpret_cast<const int*>(src);
+    int* d = reinterpret_cast<int*>(&dst);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = __builtin_amdgcn_readfirstlane(s[i]);
+    return dst;
+}
+
 // Work items are claimed per wave, so that a wave whose rays run long does
 // not hold up the other waves of its workgroup; each wave has its own slot of
 // staged BVH nodes, its own surface tallies and its own ray cursor in LDS.
@@ -98,7 +114,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
     while (true) {
         unsigned claimed = 0u;
         if (lane == 0) claimed = atomicAdd(&hdr->queue_head, 1u);
-        const int item = (int)__shfl(claimed, 0);
+        // readfirstlane (not a shuffle) makes the item provably wave-uniform,
+        // so the trace's argument struct below is read with scalar loads into
+        // SGPRs instead of occupying some 60 VGPRs
+        const int item = (int)__builtin_amdgcn_readfirstlane(claimed);
         if (item >= total) break;  // wave-uniform: every wave exits once the queue is drained
         int lo = 0, hi = num_traces - 1;  // last trace with first_packet <= item
         while (lo < hi) {
@@ -109,14 +128,18 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
                 hi = mid - 1;
         }
         const BatchTrace& bt = traces[lo];
-        SceneView sv = bt.sv;
+        SceneView sv = uniform_copy(&bt.sv);
+        const RayParams p = uniform_copy(&bt.p);
+        const Outputs out = uniform_copy(&bt.out);
+        int* const error_flag = uniform_copy(&bt.error_flag);
+        const int first_packet = __builtin_amdgcn_readfirstlane(bt.first_packet);
         sv.lds_nodes = min(lds_nodes_cap, sv.use_bvh ? sv.num_nodes : 0);
         if (sv.nodes != staged || sv.lds_nodes != staged_count) {
             for (int i = lane; i < sv.lds_nodes * 8; i += 64) my_nodes[i] = sv.nodes[i];
             staged = sv.nodes;
             staged_count = sv.lds_nodes;
         }
-        const bool lds_surf = bt.lds_surf != 0;
+        const bool lds_surf = __builtin_amdgcn_readfirstlane(bt.lds_surf) != 0;
         if (lds_surf) {
             for (int i = lane; i < sv.num_surfaces; i += 64) {
                 my_hits[i] = 0;
@@ -127,8 +150,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
         if (lane == 0) s_cursor[wave] = 0u;
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        const int begin = (item - bt.first_packet) * packet_rays;
-        const int count = min(packet_rays, bt.p.num_rays * bt.p.groups - begin);
+        const int begin = (item - first_packet) * packet_rays;
+        const int count = min(packet_rays, p.num_rays * p.groups - begin);
         Lds lds;
         lds.stack = s_stack;
         lds.queue = s_queue;
@@ -138,15 +161,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
         lds.contrib = lds_surf ? my_contrib : nullptr;
         lds.senergy = lds_surf ? my_energy : nullptr;
         LaneStats st;
-        trace_rays_refill<WG, false>(sv, bt.p, bt.out, lds, &s_cursor[wave], begin, count, st, bt.error_flag);
-        flush_stats(st, bt.p, bt.out);
+        trace_rays_refill<WG, false>(sv, p, out, lds, &s_cursor[wave], begin, count, st, error_flag);
+        flush_stats(st, p, out);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (lds_surf) {
             for (int i = lane; i < sv.num_surfaces; i += 64) {
-                if (my_hits[i]) atomicAdd(&bt.out.hit_counts[i], my_hits[i]);
-                if (my_contrib[i]) atomicAdd(&bt.out.contrib_counts[i], my_contrib[i]);
-                if (my_energy[i] != 0.0f) atomicAdd(&bt.out.surface_energy[i], my_energy[i]);
+                if (my_hits[i]) atomicAdd(&out.hit_counts[i], my_hits[i]);
+                if (my_contrib[i]) atomicAdd(&out.contrib_counts[i], my_contrib[i]);
+                if (my_energy[i] != 0.0f) atomicAdd(&out.surface_energy[i], my_energy[i]);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
