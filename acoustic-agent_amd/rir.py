@@ -1,0 +1,171 @@
+"""Echogram -> band impulse responses, on the host.
+
+The traced field is turned into sound the way the reference does after its
+tracer (acoustic_agent/steam_rt.py:2205-2232, 3070-3083, 4123-4212,
+2824-2851):
+
+* ``reconstruct_band_irs``: one white-noise sequence (``default_rng(seed +
+  7)``, uniform in [-1, 1)), shared by the six bands, is scaled per band by
+  the amplitude envelope ``sqrt(E / 4 pi)`` of the echogram (bins below
+  1e-9 are silent), interpolated linearly over the samples of each bin from
+  the previous bin's amplitude, times the air-absorption amplitude
+  ``exp(-m_b c t)`` at the bin's centre time (plus the direct delay); the
+  band signals are then band-limited.
+* ``bandlimit_band_signals``: a complementary crossover at 176.78, 353.55,
+  707.11, 1414.2 and 2828.4 Hz.  Band 0 is the low-pass at the first
+  crossover, band i the difference of the low-passes at crossovers i and
+  i-1, the top band the input minus the low-pass at the last crossover, so
+  that the six outputs of six equal inputs sum to that input.  Each low-pass
+  is an 8th-order Butterworth: four biquad sections (RBJ low-pass form, Q of
+  section k = 1 / (2 cos((2k+1) pi / 16))), run in float64 with each
+  section's output rounded to float32, as the reference's biquad does.
+* ``estimate_reconstructed_reverb_times``: RT60 of rendered band signals by
+  the same Schroeder fit as the echogram RT60, on the energy in bins of
+  ``rt_bin_duration_s``.
+
+This is host post-processing of the tracer's output (a few milliseconds per
+RIR); it runs on NumPy/SciPy and needs no GPU.
+"""
+from __future__ import annotations
+
+import math
+from typing import Any, Mapping
+
+import numpy as np
+from scipy.signal import lfilter
+
+from .models import FREQUENCY_BANDS, SimConfig
+
+_NUM_BANDS = len(FREQUENCY_BANDS)
+_ENERGY_THRESHOLD = 1e-9  # steam_rt.py:39
+_CROSSOVERS_HZ = (176.78, 353.55, 707.11, 1414.2, 2828.4)  # steam_rt.py:4125
+
+
+def _band_edges(fs: int) -> list[tuple[float, float]]:
+    """(lo, hi) of each band: the crossovers between 0 and Nyquist (steam_rt.py:4123-4127)."""
+    bounds = [0.0, *_CROSSOVERS_HZ, fs * 0.5]
+    return [(bounds[i], bounds[i + 1]) for i in range(_NUM_BANDS)]
+
+
+def _apply_biquad(values: np.ndarray, b0: float, b1: float, b2: float, a1: float, a2: float) -> np.ndarray:
+    """y[n] = b0 x[n] + b1 x[n-1] + b2 x[n-2] - a1 y[n-1] - a2 y[n-2] from rest,
+    in float64, returned as float32 (steam_rt.py:4133-4145, 4201-4212)."""
+    x = np.asarray(values, dtype=np.float64).reshape(-1)
+    if x.size == 0:
+        return x.astype(np.float32)
+    return lfilter([b0, b1, b2], [1.0, a1, a2], x).astype(np.float32)
+
+
+def _band_limit(signal: np.ndarray, lo: float, hi: float, fs: int, *, mode: str = "bandpass") -> np.ndarray:
+    """RBJ biquad filters (steam_rt.py:4148-4198): ``lowpass`` is an 8th-order
+    Butterworth at ``hi`` (four sections), ``highpass`` a Q=0.707 section at
+    ``lo``, ``bandpass`` one constant-peak section between ``lo`` and ``hi``."""
+    values = np.asarray(signal, dtype=np.float64).reshape(-1)
+    if values.size == 0:
+        return values.astype(np.float32)
+    rate = max(1, int(fs))
+    nyquist = 0.5 * rate
+    if mode == "lowpass":
+        cutoff = float(np.clip(float(hi), 1e-3, nyquist * 0.999))
+        w0 = 2.0 * math.pi * cutoff / rate
+        cw0, sw0 = math.cos(w0), math.sin(w0)
+        out = values
+        order = 8
+        for section in range(order // 2):
+            q = 1.0 / (2.0 * math.cos((2.0 * section + 1.0) * math.pi / (2.0 * order)))
+            alpha = sw0 / (2.0 * q)
+            a0 = 1.0 + alpha
+            b0 = ((1.0 - cw0) / 2.0) / a0
+            out = _apply_biquad(out, b0, (1.0 - cw0) / a0, b0, (-2.0 * cw0) / a0, (1.0 - alpha) / a0)
+        return out
+    if mode == "highpass":
+        cutoff = float(np.clip(float(lo), 1e-3, nyquist * 0.999))
+        w0 = 2.0 * math.pi * cutoff / rate
+        cw0, sw0 = math.cos(w0), math.sin(w0)
+        alpha = sw0 / (2.0 * 0.707)
+        a0 = 1.0 + alpha
+        b0 = ((1.0 + cw0) / 2.0) / a0
+        return _apply_biquad(values, b0, (-(1.0 + cw0)) / a0, b0, (-2.0 * cw0) / a0, (1.0 - alpha) / a0)
+    if mode != "bandpass":
+        raise ValueError(f"unknown filter mode {mode!r}")
+    lower = float(np.clip(float(lo), 1e-3, nyquist * 0.998))
+    upper = float(np.clip(float(hi), lower + 1e-3, nyquist * 0.999))
+    centre = math.sqrt(max(lower * upper, 1e-12))
+    w0 = 2.0 * math.pi * centre / rate
+    cw0, sw0 = math.cos(w0), math.sin(w0)
+    alpha = sw0 * ((upper - lower) / centre) / 2.0
+    a0 = 1.0 + alpha
+    return _apply_biquad(values, alpha / a0, 0.0, -alpha / a0, (-2.0 * cw0) / a0, (1.0 - alpha) / a0)
+
+
+def bandlimit_band_signals(signals: np.ndarray, fs: int) -> np.ndarray:
+    """Complementary six-band crossover of [6, n] band signals -> float32 [6, n]
+    (steam_rt.py:3070-3083)."""
+    values = np.asarray(signals, dtype=np.float64)
+    if values.ndim != 2 or values.shape[0] != _NUM_BANDS:
+        raise ValueError(f"expected {_NUM_BANDS} band signals, got shape {values.shape}")
+    crossovers = [edge[1] for edge in _band_edges(fs)[:-1]]
+    out = np.zeros_like(values, dtype=np.float32)
+    out[0] = _band_limit(values[0], 0.0, crossovers[0], fs, mode="lowpass")
+    for b in range(1, _NUM_BANDS - 1):
+        upper = _band_limit(values[b], 0.0, crossovers[b], fs, mode="lowpass")
+        lower = _band_limit(values[b], 0.0, crossovers[b - 1], fs, mode="lowpass")
+        out[b] = upper - lower
+    out[-1] = values[-1] - _band_limit(values[-1], 0.0, crossovers[-1], fs, mode="lowpass")
+    return out
+
+
+def _air_absorption_amplitude(coefficient_per_m: float, travel_time_s: float, speed_m_s: float) -> float:
+    """exp(-m d) for d = c t (steam_rt.py:2191-2193)."""
+    distance_m = max(0.0, float(travel_time_s)) * max(0.0, float(speed_m_s))
+    return math.exp(-max(0.0, float(coefficient_per_m)) * distance_m)
+
+
+def reconstruct_band_irs(field: Mapping[str, Any], config: SimConfig) -> np.ndarray:
+    """Band impulse responses [6, K * ceil(bin * fs)] (float32) of an echogram
+    field (steam_rt.py:2205-2232)."""
+    from . import steam_rt as _rt  # the absorption table lives there (steam_rt re-exports this module)
+
+    echogram = np.asarray(field["echogram"], dtype=np.float64)
+    num_bins = int(field["num_bins"])
+    fs = int(config.fs)
+    spb = max(1, int(math.ceil(float(field["bin_duration_s"]) * fs)))
+    num_samples = num_bins * spb
+    white = np.random.default_rng(config.seed + 7).uniform(-1.0, 1.0, size=num_samples).astype(np.float64)
+    w = np.arange(spb, dtype=np.float64) / spb
+    direct_delay_s = max(0.0, float(field.get("direct_delay_s", 0.0)))
+    # distance travelled at each bin's centre time (>= 0), for the absorption
+    centre_m = (direct_delay_s + (np.arange(num_bins, dtype=np.float64) + 0.5) * spb / fs) * max(0.0, float(config.c))
+    raw = np.zeros((_NUM_BANDS, num_samples), dtype=np.float64)
+    for bi, band in enumerate(FREQUENCY_BANDS):
+        coeff = _rt.AIR_ABSORPTION_NP_PER_M[band]
+        amps = np.sqrt(np.clip(echogram[bi], 0.0, None) / (4.0 * math.pi))
+        amps[echogram[bi] < _ENERGY_THRESHOLD] = 0.0
+        prev = np.concatenate([amps[:1], amps[:-1]])
+        seg = (1.0 - w)[None, :] * prev[:, None] + w[None, :] * amps[:, None]
+        air = np.exp(-max(0.0, float(coeff)) * centre_m)  # _air_absorption_amplitude per bin
+        raw[bi] = (seg * air[:, None]).reshape(-1) * white
+    return bandlimit_band_signals(raw, fs)
+
+
+def estimate_reconstructed_reverb_times(band_rirs: np.ndarray, config: SimConfig) -> dict[str, float]:
+    """Band RT60 of rendered band signals [6, n]: energy per bin of
+    ``rt_bin_duration_s``, then the Schroeder fit (steam_rt.py:2824-2851)."""
+    from . import steam_rt as _rt
+
+    fs = max(1, int(config.fs))
+    signals = np.asarray(band_rirs, dtype=np.float64)
+    if signals.ndim != 2 or signals.shape[0] != _NUM_BANDS:
+        return {band: 0.0 for band in FREQUENCY_BANDS}
+    bin_samples = max(1, int(round(float(config.rt_bin_duration_s) * fs)))
+    n = int(signals.shape[1])
+    num_bins = max(1, int(math.ceil(n / bin_samples)))
+    squared = np.zeros((_NUM_BANDS, num_bins * bin_samples), dtype=np.float64)
+    squared[:, :n] = np.square(signals)
+    energy = squared.reshape(_NUM_BANDS, num_bins, bin_samples).sum(axis=2)
+    return {band: _rt._schroeder_fit_rt60(energy[bi], bin_samples / fs) for bi, band in enumerate(FREQUENCY_BANDS)}
+
+
+def render_rir(field: Mapping[str, Any], config: SimConfig) -> np.ndarray:
+    """Mono impulse response: the sum of the reconstructed band signals."""
+    return np.sum(reconstruct_band_irs(field, config), axis=0, dtype=np.float64).astype(np.float32)

@@ -243,3 +243,11 @@ def estimate_early_decay_times(field: Mapping[str, Any], config: SimConfig) -> d
     bin_dur = float(field["bin_duration_s"])
     return {band: _schroeder_fit_edt(w, bin_dur, min_total_energy=1e-4)
             for band, w in zip(FREQUENCY_BANDS, _weighted_bands(field, config))}
+
+
+# echogram -> band impulse responses (host), under the reference's names
+from .rir import (  # noqa: E402,F401
+    bandlimit_band_signals,
+    estimate_reconstructed_reverb_times,
+    reconstruct_band_irs,
+)

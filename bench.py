@@ -32,7 +32,12 @@ load_package()
 from acoustic_agent_amd import hip_backend, sampling, scenes  # noqa: E402
 from acoustic_agent_amd import _lib  # noqa: E402
 from acoustic_agent_amd.models import SimConfig  # noqa: E402
-from acoustic_agent_amd.steam_rt import estimate_early_decay_times, estimate_reverb_times  # noqa: E402
+from acoustic_agent_amd.steam_rt import (  # noqa: E402
+    estimate_early_decay_times,
+    estimate_reconstructed_reverb_times,
+    estimate_reverb_times,
+    reconstruct_band_irs,
+)
 
 C = 343.0
 HBM_PEAK_GBS = 8000.0
@@ -308,6 +313,21 @@ def main() -> int:
         "kernel_ms_avg": round(avg_kernel_s * 1000.0, 4),
         "gpu_counters": counters,
         "scene": {k: scene.info[k] for k in ("num_nodes", "depth", "canonical_nodes", "device_bytes")},
+    }
+
+    # host post-processing of the same field: band impulse responses and their RT60
+    field = {"echogram": gpu_echogram, "num_bins": K, "bin_duration_s": bin_dur, "direct_delay_s": delay}
+    cfg = SimConfig()
+    t_r = time.perf_counter()
+    band_irs = reconstruct_band_irs(field, cfg)
+    rendered = estimate_reconstructed_reverb_times(band_irs, cfg)
+    rir_ms = (time.perf_counter() - t_r) * 1000.0
+    traced = estimate_reverb_times(field, cfg)
+    result["host_rir"] = {
+        "fs": cfg.fs, "samples": int(band_irs.shape[1]), "ms": round(rir_ms, 2),
+        "rt60_rendered_s": {k: round(v, 5) for k, v in rendered.items()},
+        "rt60_rendered_vs_traced_max_rel": round(max(abs(rendered[k] - traced[k]) / traced[k]
+                                                     for k in traced if traced[k] > 0), 4),
     }
 
     roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,

@@ -1,0 +1,151 @@
+"""Echogram -> band impulse responses (host): reconstruction, crossover, RT60.
+
+Pinned by the reference's own known-answer tests
+(tests/test_steam_audio_parity.py:287-326: perfect reconstruction, band-centre
+calibration, coherent noise) and by closed forms: the 8th-order Butterworth
+magnitude of each low-pass, a brute-force biquad recursion, the amplitude
+envelope and air absorption of the reconstruction, and the RT60 of an exact
+exponential decay.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from acoustic_agent_amd import rir, scenes, steam_rt
+from acoustic_agent_amd.models import FREQUENCY_BANDS, SimConfig
+from acoustic_agent_amd.steam_rt import (
+    bandlimit_band_signals,
+    estimate_reconstructed_reverb_times,
+    estimate_reverb_times,
+    reconstruct_band_irs,
+)
+
+from helpers import Case, run_oracle
+
+
+def test_six_band_synthesis_is_perfectly_reconstructing_for_equal_inputs():
+    # tests/test_steam_audio_parity.py:287-293
+    signal = np.random.default_rng(42).standard_normal(4096).astype(np.float32)
+    bands = np.tile(signal, (len(FREQUENCY_BANDS), 1))
+    reconstructed = np.sum(bandlimit_band_signals(bands, 16000), axis=0)
+    np.testing.assert_allclose(reconstructed, signal, rtol=1e-6, atol=2e-6)
+
+
+def test_six_band_synthesis_is_calibrated_at_band_centers():
+    # tests/test_steam_audio_parity.py:296-309
+    fs, length = 16000, 65536
+    impulse = np.zeros(length, dtype=np.float32)
+    impulse[0] = 1.0
+    branches = bandlimit_band_signals(np.tile(impulse, (len(FREQUENCY_BANDS), 1)), fs)
+    spectra = np.abs(np.fft.rfft(branches, axis=1))
+    freqs = np.fft.rfftfreq(length, 1.0 / fs)
+    for bi, band in enumerate(FREQUENCY_BANDS):
+        k = int(np.argmin(np.abs(freqs - float(band))))
+        assert abs(20.0 * math.log10(max(float(spectra[bi, k]), 1e-20))) < 0.25, band
+
+
+def test_reconstructor_uses_coherent_noise_across_bands(monkeypatch):
+    # tests/test_steam_audio_parity.py:312-326
+    for band in FREQUENCY_BANDS:
+        monkeypatch.setitem(steam_rt.AIR_ABSORPTION_NP_PER_M, band, 0.0)
+    config = SimConfig(fs=16000, duration_s=0.04, seed=91)
+    field = {"echogram": np.ones((len(FREQUENCY_BANDS), 4)), "num_bins": 4, "bin_duration_s": 0.01}
+    reconstructed = np.sum(reconstruct_band_irs(field, config), axis=0)
+    white = np.random.default_rng(config.seed + 7).uniform(-1.0, 1.0, size=reconstructed.size)
+    np.testing.assert_allclose(reconstructed, white / math.sqrt(4.0 * math.pi), rtol=1e-5, atol=2e-6)
+
+
+def test_lowpass_is_8th_order_butterworth():
+    # bilinear transform prewarped at fc: |H(f)|^2 = 1 / (1 + (tan(pi f/fs) / tan(pi fc/fs))^16)
+    fs, length, fc = 48000, 1 << 17, 1000.0
+    impulse = np.zeros(length)
+    impulse[0] = 1.0
+    h = np.abs(np.fft.rfft(rir._band_limit(impulse, 0.0, fc, fs, mode="lowpass").astype(np.float64)))
+    freqs = np.fft.rfftfreq(length, 1.0 / fs)
+    for f in (0.5 * fc, fc, 2 * fc, 4 * fc):
+        k = int(np.argmin(np.abs(freqs - f)))
+        ratio = math.tan(math.pi * freqs[k] / fs) / math.tan(math.pi * fc / fs)
+        want_db = -10 * math.log10(1 + ratio ** 16)
+        got = 20 * math.log10(h[k])
+        assert abs(got - want_db) < 0.02, (f, got, want_db)
+
+
+def test_biquad_equals_the_direct_form_recursion():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(300)
+    b0, b1, b2, a1, a2 = 0.2, -0.1, 0.05, -0.6, 0.2
+    want = np.zeros(x.size)
+    x1 = x2 = y1 = y2 = 0.0
+    for i, x0 in enumerate(x):
+        y0 = b0 * x0 + b1 * x1 + b2 * x2 - a1 * y1 - a2 * y2
+        want[i] = y0
+        x2, x1, y2, y1 = x1, x0, y1, y0
+    got = rir._apply_biquad(x, b0, b1, b2, a1, a2)
+    assert got.dtype == np.float32
+    np.testing.assert_allclose(got, want.astype(np.float32), rtol=2e-7, atol=1e-7)
+
+
+def _envelope(amps: np.ndarray, spb: int) -> np.ndarray:
+    """Sample n of bin b: (1 - w) a[b-1] + w a[b], w = (n mod spb) / spb, a[-1] = a[0]."""
+    out = np.empty(amps.size * spb)
+    for n in range(out.size):
+        b, r = divmod(n, spb)
+        w = r / spb
+        out[n] = (1.0 - w) * amps[max(b - 1, 0)] + w * amps[b]
+    return out
+
+
+@pytest.mark.parametrize("coeff", [0.0, 0.004])
+def test_reconstruction_envelope_and_air_absorption(monkeypatch, coeff):
+    # with the same echogram and absorption in every band the crossover sums
+    # back to the unfiltered signal, which is noise x envelope x air absorption
+    for band in FREQUENCY_BANDS:
+        monkeypatch.setitem(steam_rt.AIR_ABSORPTION_NP_PER_M, band, coeff)
+    config = SimConfig(fs=8000, seed=5)
+    rng = np.random.default_rng(11)
+    e = rng.uniform(0.0, 2.0, 12)
+    e[[3, 7]] = 5e-10  # below the 1e-9 threshold: silent bins
+    field = {"echogram": np.tile(e, (6, 1)), "num_bins": 12, "bin_duration_s": 0.0025, "direct_delay_s": 0.013}
+    spb = 20
+    total = np.sum(reconstruct_band_irs(field, config).astype(np.float64), axis=0)
+    amps = np.sqrt(e / (4 * math.pi))
+    amps[e < 1e-9] = 0.0
+    centre = 0.013 + (np.arange(12) + 0.5) * spb / 8000
+    air = np.repeat(np.exp(-coeff * centre * config.c), spb)
+    white = np.random.default_rng(config.seed + 7).uniform(-1.0, 1.0, 12 * spb)
+    np.testing.assert_allclose(total, white * _envelope(amps, spb) * air, rtol=2e-5, atol=2e-6)
+
+
+def test_reconstructed_rt60_of_an_exponential_decay():
+    fs, T = 16000, 0.9
+    n = 2 * fs
+    t = np.arange(n) / fs
+    rng = np.random.default_rng(83)
+    bands = np.vstack([rng.uniform(-1, 1, n) * 10 ** (-3 * t / T) for _ in FREQUENCY_BANDS])
+    rt = estimate_reconstructed_reverb_times(bands, SimConfig(fs=fs))
+    for band in FREQUENCY_BANDS:
+        assert rt[band] == pytest.approx(T, rel=0.02)
+    assert estimate_reconstructed_reverb_times(np.zeros((5, 10)), SimConfig()) == {b: 0.0 for b in FREQUENCY_BANDS}
+
+
+def test_traced_field_to_rir_keeps_the_band_rt60():
+    # a traced shoebox field (FP64 oracle) rendered to band signals: the RT60
+    # of the rendered signals follows the echogram's band RT60
+    w = scenes.shoebox_workload(seed=0)
+    case = Case(w, num_rays=8192, num_bounces=80, num_bins=1000, seed=3)
+    ref = run_oracle(case, "float64")
+    field = {"echogram": ref["echogram"], "num_bins": case.num_bins, "bin_duration_s": case.bin_dur,
+             "direct_delay_s": case.direct_delay(w.source)}
+    config = SimConfig(fs=16000, seed=3)
+    traced = estimate_reverb_times(field, config)
+    bands = reconstruct_band_irs(field, config)
+    assert bands.shape == (6, case.num_bins * 16) and bands.dtype == np.float32
+    rendered = estimate_reconstructed_reverb_times(bands, config)
+    for band in FREQUENCY_BANDS:
+        assert traced[band] > 0
+        assert rendered[band] == pytest.approx(traced[band], rel=0.15), (band, rendered[band], traced[band])
+    mono = rir.render_rir(field, config)
+    np.testing.assert_allclose(mono, np.sum(bands.astype(np.float64), axis=0), rtol=1e-6, atol=1e-9)
