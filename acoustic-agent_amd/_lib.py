@@ -140,6 +140,7 @@ _SIGNATURES = {
     "aa_build_bvh_sah": (_I32, [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]),
     "aa_scene_create": (_I32, [_I32, _P, _P, _P, _I32, _P, _P, _I32, _I32, ctypes.POINTER(_P)]),
     "aa_scene_info": (_I32, [_P, ctypes.POINTER(SceneInfo)]),
+    "aa_scene_hash": (_I32, [_P, _P, _P, _I32, _P, _P, _I32, _I32, ctypes.POINTER(ctypes.c_uint64)]),
     "aa_scene_release": (_I32, [_P]),
     "aa_trace": (_I32, [_P, ctypes.POINTER(TraceParams), _P, _P, _P, _P, _P, ctypes.POINTER(TraceOutputs), _P]),
     "aa_trace_plain": (_I32, [_P, ctypes.POINTER(TraceParams), _P, _P, _P, _P, _P, ctypes.POINTER(TraceOutputs),

@@ -299,6 +299,20 @@ uint64_t fnv1a(uint64_t h, const void* data, size_t bytes) {
     return h;
 }
 
+// FNV-1a 64 over the scene inputs in argument order: the key of a scene in
+// the caches and on disk.
+uint64_t scene_hash(const float* tri_vertices, const int32_t* tri_surface, const float* tri_normals,
+                    int32_t num_triangles, const float* reflection, const float* scattering, int32_t num_surfaces,
+                    int32_t leaf_size) {
+    uint64_t h = 1469598103934665603ull;
+    h = fnv1a(h, tri_vertices, (size_t)num_triangles * 9 * sizeof(float));
+    h = fnv1a(h, tri_surface, (size_t)num_triangles * sizeof(int32_t));
+    h = fnv1a(h, tri_normals, (size_t)num_triangles * 3 * sizeof(float));
+    h = fnv1a(h, reflection, (size_t)num_surfaces * 6 * sizeof(float));
+    h = fnv1a(h, scattering, (size_t)num_surfaces * sizeof(float));
+    return fnv1a(h, &leaf_size, sizeof(leaf_size));
+}
+
 void free_scene(aa_scene_s* s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
@@ -398,6 +412,18 @@ int aa_build_bvh_sah(const double* bounds_min, const double* bounds_max, int64_t
                      int64_t* primitives, int64_t* stats) {
     return build_bvh_common(true, bounds_min, bounds_max, n, leaf_size, node_min, node_max, start, count, escape,
                             primitives, stats);
+}
+
+int aa_scene_hash(const float* tri_vertices, const int32_t* tri_surface, const float* tri_normals,
+                  int32_t num_triangles, const float* reflection, const float* scattering, int32_t num_surfaces,
+                  int32_t leaf_size, uint64_t* hash_out) {
+    if (!hash_out || num_triangles < 0 || num_surfaces < 0 ||
+        (num_triangles > 0 && (!tri_vertices || !tri_surface || !tri_normals)) ||
+        (num_surfaces > 0 && (!reflection || !scattering)))
+        return fail(AA_ERR_INVALID, "aa_scene_hash: NULL array or negative count");
+    *hash_out = scene_hash(tri_vertices, tri_surface, tri_normals, num_triangles, reflection, scattering,
+                           num_surfaces, leaf_size);
+    return AA_OK;
 }
 
 int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tri_surface, const float* tri_normals,
@@ -514,14 +540,8 @@ int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tr
         normals[(size_t)k * 4 + 2] = tri_normals[(size_t)k * 3 + 2];
         normals[(size_t)k * 4 + 3] = 0.0f;
     }
-    uint64_t h = 1469598103934665603ull;
-    h = fnv1a(h, tri_vertices, (size_t)num_triangles * 9 * sizeof(float));
-    h = fnv1a(h, tri_surface, (size_t)num_triangles * sizeof(int32_t));
-    h = fnv1a(h, tri_normals, (size_t)num_triangles * 3 * sizeof(float));
-    h = fnv1a(h, reflection, (size_t)num_surfaces * 6 * sizeof(float));
-    h = fnv1a(h, scattering, (size_t)num_surfaces * sizeof(float));
-    h = fnv1a(h, &leaf_size, sizeof(leaf_size));
-    s->content_hash = h;
+    s->content_hash = scene_hash(tri_vertices, tri_surface, tri_normals, num_triangles, reflection, scattering,
+                                 num_surfaces, leaf_size);
 
     hipError_t e = hipSetDevice(device);
     const size_t node_bytes = s->wide.nodes.size() * sizeof(aa::WideNode);

@@ -204,6 +204,8 @@ def _device_input(value: np.ndarray, dtype: Any, device: int, stats: dict[str, i
             stats["hits"] += 1
             return cached[1]
     converted = np.ascontiguousarray(host, dtype=dtype)
+    if not converted.flags.writeable:  # the reference marks its arrays read-only; torch wants writable memory
+        converted = converted.copy()
     tensor = torch.from_numpy(converted).to(f"cuda:{device}")
     size = int(converted.nbytes)
     with _CACHE_LOCK:

@@ -103,6 +103,15 @@ int aa_scene_create(int32_t device, const float *tri_vertices /*[T,9]*/, const i
                     const float *tri_normals /*[T,3]*/, int32_t num_triangles, const float *reflection /*[S,6]*/,
                     const float *scattering /*[S]*/, int32_t num_surfaces, int32_t leaf_size, aa_scene_t *out);
 int aa_scene_info(aa_scene_t scene, aa_scene_info_t *info);
+
+/* The content_hash aa_scene_create stores, computed on the host without a
+ * device: FNV-1a 64 over the float32/int32 inputs in argument order, then
+ * leaf_size.  It keys scenes in caches and in scene files.  (No reference
+ * counterpart: steam_rt_cuda.py:229-261 keys its device cache by host
+ * pointer, shape and dtype.) */
+int aa_scene_hash(const float *tri_vertices, const int32_t *tri_surface, const float *tri_normals,
+                  int32_t num_triangles, const float *reflection, const float *scattering, int32_t num_surfaces,
+                  int32_t leaf_size, uint64_t *hash_out);
 int aa_scene_release(aa_scene_t scene);
 
 /* ---- one trace (replaces trace_energy_field_cuda, steam_rt_cuda.py:44-226) ---- */

@@ -118,3 +118,32 @@ def test_batch_ambisonics_equal_single_traces():
         if i % 2:
             assert got[i]["ambisonic"].shape == (1, 6, 4, 1000)
             assert rel_l1(got[i]["ambisonic"], single[i]["ambisonic"]) <= 1e-5
+
+
+def test_planned_motion_equals_frame_by_frame_traces():
+    # the motion planner's multi-source trace equals one trace per frame
+    from acoustic_agent_amd.mic import microphone_array
+    from acoustic_agent_amd.models import SimConfig
+    from acoustic_agent_amd.plan import motion_traces
+
+    w = scenes.l_plan_workload(0, target_triangles=20000)
+    ds = _scene(w)
+    cfg = SimConfig(rt_num_rays=3000, rt_num_bounces=60, rt_duration_s=1.0, duration_s=1.0, rt_bin_duration_s=1e-3,
+                    seed=21)
+    path = scenes.moving_source_path(4, keyframes=5)
+    frames = [{"source": p, "receiver": w.listener} for p in path]
+    plan = motion_traces(ds, frames, cfg, microphone_array("linear", count=2, spacing_m=0.1))
+    assert len(plan.traces) == 2
+    engine = BatchEngine(0, max_traces=8)
+    got = engine.submit(plan.traces).wait()
+    engine.close()
+    for f, frame in enumerate(frames):
+        for t, k in plan.frames[f]:
+            tr = plan.traces[t]
+            single = TraceRequest(scene=ds, listener=tr.listener, sources=np.asarray(frame["source"])[None, :],
+                                  num_rays=tr.num_rays, num_bounces=tr.num_bounces, duration_s=tr.duration_s,
+                                  bin_dur=tr.bin_dur, seed=tr.seed)
+            (one,), _ = trace_sequential([single])
+            assert rel_l1(got[t]["echogram"][k], one["echogram"][0]) <= 1e-5
+            np.testing.assert_array_equal(got[t]["hit_counts"][k], one["hit_counts"][0])
+            assert int(got[t]["active_count"][k]) == int(one["active_count"][0])

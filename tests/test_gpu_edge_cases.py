@@ -99,3 +99,21 @@ def test_sixty_four_sources_equal_separate_traces(shoebox):
         assert np.array_equal(multi["hit_counts"][k], single["hit_counts"])
         assert int(multi["active_count"][k]) == single["active_count"]
         assert int(multi["actual_bounces"][k]) == single["actual_bounces"]
+
+
+def test_scene_file_loads_as_the_same_device_scene(shoebox, tmp_path):
+    # a scene written to disk and loaded back is the same scene: same content
+    # hash on the device, same trace
+    from acoustic_agent_amd.scene_file import device_scene_from_file, save_scene
+
+    a = shoebox.scene.kernel_arrays()
+    path = tmp_path / "box.aascene"
+    h = save_scene(path, shoebox.mesh, a["reflection"], a["scattering"])
+    loaded = device_scene_from_file(path)
+    assert int(loaded.info["content_hash"]) == h
+    case = Case(shoebox, num_rays=2048, num_bounces=30, num_bins=1000, seed=8, rng="counter")
+    base = run_gpu(case)
+    again = run_gpu(case, scene=loaded)
+    assert np.array_equal(base["hit_counts"], again["hit_counts"])
+    assert rel_l1(again["echogram"], base["echogram"]) <= 1e-5
+    loaded.close()
