@@ -149,6 +149,7 @@ _SIGNATURES = {
     "aa_batch_create": (_I32, [_I32, _I32, ctypes.POINTER(_P)]),
     "aa_batch_submit": (_I32, [_P, ctypes.POINTER(BatchTrace), _I32, ctypes.POINTER(BatchTuning), _P]),
     "aa_batch_release": (_I32, [_P]),
+    "aa_fdn_impulse": (_I32, [_P, _I32, _P, _I32, _I64, _P, _I32, _P, _P]),
 }
 
 _lock = threading.Lock()

@@ -15,7 +15,7 @@ PACKAGE_DIR = Path(__file__).resolve().parent
 REPO_DIR = PACKAGE_DIR.parent
 CSRC = PACKAGE_DIR / "csrc"
 LIB_PATH = PACKAGE_DIR / "libaa_tracer.so"
-SOURCES = ("aa_tracer.hip", "aa_bvh.cpp", "aa_batch.hip")
+SOURCES = ("aa_tracer.hip", "aa_bvh.cpp", "aa_batch.hip", "aa_fdn.cpp")
 
 # Device code: IEEE division/sqrt, no contraction (the binary32 trajectory
 # must round like the CPU restatement), no packed-FP32 instructions.

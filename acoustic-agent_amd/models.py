@@ -1,7 +1,8 @@
 """Tracer configuration contract.
 
-Mirrors the tracer-facing part of the reference's ``SimConfig``
-(acoustic_agent/models.py:60-112) with the same field names and defaults, plus
+Mirrors the tracer- and late-tail-facing part of the reference's
+``SimConfig`` (acoustic_agent/models.py:60-112) with the same field names and
+defaults, plus
 the fields of the HIP backend (``rt_hip_device``, ``rt_rng``,
 ``rt_tessellation_max_edge_m``).  The six bands are those of
 acoustic_agent/models.py:9.
@@ -19,6 +20,9 @@ class SimConfig:
     c: float = 343.0
     duration_s: float = 2.0
     seed: int = 1729
+    late_tail: bool = True
+    hybrid_transition_s: float = 1.0
+    hybrid_overlap_fraction: float = 0.25
     rt_accelerator: str = "hip"
     rt_precision: str = "float32"
     rt_cuda_device: int = 0

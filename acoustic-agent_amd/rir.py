@@ -22,6 +22,8 @@ tracer (acoustic_agent/steam_rt.py:2205-2232, 3070-3083, 4123-4212,
 * ``estimate_reconstructed_reverb_times``: RT60 of rendered band signals by
   the same Schroeder fit as the echogram RT60, on the energy in bins of
   ``rt_bin_duration_s``.
+* ``render_band_rirs`` / ``render_rir``: the reconstruction with the FDN late
+  tail crossfaded in (``fdn.py``) at the field's band RT60, summed to mono.
 
 This is host post-processing of the tracer's output (a few milliseconds per
 RIR); it runs on NumPy/SciPy and needs no GPU.
@@ -166,6 +168,21 @@ def estimate_reconstructed_reverb_times(band_rirs: np.ndarray, config: SimConfig
     return {band: _rt._schroeder_fit_rt60(energy[bi], bin_samples / fs) for bi, band in enumerate(FREQUENCY_BANDS)}
 
 
+def render_band_rirs(field: Mapping[str, Any], config: SimConfig,
+                     rt60_bands: Mapping[str, float] | None = None) -> tuple[np.ndarray, dict[str, Any]]:
+    """Band impulse responses [6, n] of a traced field with the FDN late tail
+    (``config.late_tail``) at the field's band RT60 (or ``rt60_bands``), and
+    the late-tail metadata."""
+    from . import steam_rt as _rt
+    from .fdn import _render_parametric_fdn_late_reverb
+
+    traced = reconstruct_band_irs(field, config)
+    rt60 = dict(rt60_bands) if rt60_bands is not None else _rt.estimate_reverb_times(field, config)
+    bands, _, meta = _render_parametric_fdn_late_reverb(traced, field, rt60, config)
+    return bands, meta
+
+
 def render_rir(field: Mapping[str, Any], config: SimConfig) -> np.ndarray:
-    """Mono impulse response: the sum of the reconstructed band signals."""
-    return np.sum(reconstruct_band_irs(field, config), axis=0, dtype=np.float64).astype(np.float32)
+    """Mono impulse response: the sum of the band impulse responses."""
+    bands, _ = render_band_rirs(field, config)
+    return np.sum(bands, axis=0, dtype=np.float64).astype(np.float32)

@@ -32,6 +32,7 @@ load_package()
 from acoustic_agent_amd import hip_backend, sampling, scenes  # noqa: E402
 from acoustic_agent_amd import _lib  # noqa: E402
 from acoustic_agent_amd.models import SimConfig  # noqa: E402
+from acoustic_agent_amd.rir import render_band_rirs  # noqa: E402
 from acoustic_agent_amd.steam_rt import (  # noqa: E402
     estimate_early_decay_times,
     estimate_reconstructed_reverb_times,
@@ -315,20 +316,27 @@ def main() -> int:
         "scene": {k: scene.info[k] for k in ("num_nodes", "depth", "canonical_nodes", "device_bytes")},
     }
 
-    # host post-processing of the same field: band impulse responses and their RT60
+    # host post-processing of the same field: band impulse responses with the
+    # FDN late tail (reference defaults: 16 kHz, transition at 1.0 s), and their RT60
     field = {"echogram": gpu_echogram, "num_bins": K, "bin_duration_s": bin_dur, "direct_delay_s": delay}
     cfg = SimConfig()
     t_r = time.perf_counter()
     band_irs = reconstruct_band_irs(field, cfg)
-    rendered = estimate_reconstructed_reverb_times(band_irs, cfg)
-    rir_ms = (time.perf_counter() - t_r) * 1000.0
+    t_f = time.perf_counter()
+    band_rirs, tail = render_band_rirs(field, cfg)
+    t_e = time.perf_counter()
+    rendered = estimate_reconstructed_reverb_times(band_rirs, cfg)
     traced = estimate_reverb_times(field, cfg)
     result["host_rir"] = {
-        "fs": cfg.fs, "samples": int(band_irs.shape[1]), "ms": round(rir_ms, 2),
+        "fs": cfg.fs, "samples": int(band_rirs.shape[1]),
+        "reconstruct_ms": round((t_f - t_r) * 1000.0, 2),
+        "reconstruct_and_fdn_tail_ms": round((t_e - t_f) * 1000.0, 2),
+        "late_tail_applied": bool(tail["applied"]),
         "rt60_rendered_s": {k: round(v, 5) for k, v in rendered.items()},
         "rt60_rendered_vs_traced_max_rel": round(max(abs(rendered[k] - traced[k]) / traced[k]
                                                      for k in traced if traced[k] > 0), 4),
     }
+    del band_irs
 
     roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
                 "traffic": None}

@@ -221,6 +221,17 @@ int aa_batch_submit(aa_batch_t batch, const aa_batch_trace *traces, int32_t coun
                     void *stream);
 int aa_batch_release(aa_batch_t batch);
 
+/* ---- host: late-reverberation FDN (replaces _fdn_impulse_kernel,
+ *      steam_rt.py:2235-2328) ----
+ * Impulse response of num_delays (a power of two) Hadamard-mixed delay lines
+ * of delays[] samples, each with a cascade of num_bands absorptive biquads
+ * absorptive[line][band][b0 b1 b2 a1 a2], output = mean of the lines through
+ * num_allpass Schroeder all-passes (gain 0.5) and the tone cascade
+ * tone[band][5]; binary64, out[num_samples].  Host memory only. */
+int aa_fdn_impulse(const int64_t *delays, int32_t num_delays, const double *absorptive, int32_t num_bands,
+                   int64_t num_samples, const int64_t *allpass_delays, int32_t num_allpass, const double *tone,
+                   double *out);
+
 #ifdef __cplusplus
 }
 #endif

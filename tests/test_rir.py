@@ -147,5 +147,15 @@ def test_traced_field_to_rir_keeps_the_band_rt60():
     for band in FREQUENCY_BANDS:
         assert traced[band] > 0
         assert rendered[band] == pytest.approx(traced[band], rel=0.15), (band, rendered[band], traced[band])
-    mono = rir.render_rir(field, config)
-    np.testing.assert_allclose(mono, np.sum(bands.astype(np.float64), axis=0), rtol=1e-6, atol=1e-9)
+    # without the late tail the mono RIR is the sum of the reconstructed bands
+    dry = SimConfig(fs=16000, seed=3, late_tail=False)
+    np.testing.assert_allclose(rir.render_rir(field, dry), np.sum(bands.astype(np.float64), axis=0),
+                               rtol=1e-6, atol=1e-9)
+    # with it (transition at 0.3 s) the rendered tail keeps following the traced RT60
+    hybrid = SimConfig(fs=16000, seed=3, hybrid_transition_s=0.3)
+    hb, meta = rir.render_band_rirs(field, hybrid)
+    assert meta["applied"] and meta["transition_start_s"] == pytest.approx(0.225, abs=1 / 16000)
+    rendered = estimate_reconstructed_reverb_times(hb, hybrid)
+    for band in FREQUENCY_BANDS:
+        assert rendered[band] == pytest.approx(traced[band], rel=0.15), (band, rendered[band], traced[band])
+    assert rir.render_rir(field, hybrid).shape == (case.num_bins * 16,)
