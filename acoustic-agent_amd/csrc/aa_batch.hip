@@ -100,7 +100,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
     int* s_surf = reinterpret_cast<int*>(s_nodes + (size_t)kWaves * lds_nodes_cap * 8);
     __shared__ unsigned s_cursor[kWaves];
 
-    const int wave = (int)(threadIdx.x >> 6);
+    const int wave = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar LDS addresses
     const int lane = (int)(threadIdx.x & 63);
     float4* my_nodes = s_nodes + (size_t)wave * lds_nodes_cap * 8;
     int* my_hits = s_surf + (size_t)wave * surf_cap * 3;

@@ -874,7 +874,10 @@ template <int WG, bool LDS_HIST>
 __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds,
                                   unsigned* cursor, int first, int count, LaneStats& st, int* error_flag) {
     unsigned short* stk = lds.stack + threadIdx.x;
-    float* qu = lds.queue + (size_t)(threadIdx.x >> 6) * (kQueueFields * kQueueCap);
+    // the wave's index, provably uniform (readfirstlane), so that its queue
+    // address is a scalar
+    const int wave = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    float* qu = lds.queue + (size_t)wave * (kQueueFields * kQueueCap);
     const int lane = (int)(threadIdx.x & 63);
     const unsigned long long below = (1ull << lane) - 1ull;
     RayState s;
