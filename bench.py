@@ -333,8 +333,8 @@ def main() -> int:
         "reconstruct_and_fdn_tail_ms": round((t_e - t_f) * 1000.0, 2),
         "late_tail_applied": bool(tail["applied"]),
         "rt60_rendered_s": {k: round(v, 5) for k, v in rendered.items()},
-        "rt60_rendered_vs_traced_max_rel": round(max(abs(rendered[k] - traced[k]) / traced[k]
-                                                     for k in traced if traced[k] > 0), 4),
+        "rt60_rendered_vs_traced_max_rel": max((round(abs(rendered[k] - traced[k]) / traced[k], 4)
+                                                for k in traced if traced[k] > 0), default=None),
     }
     del band_irs
 
