@@ -83,6 +83,8 @@ class TraceParams(ctypes.Structure):
         ("sources_per_lane", ctypes.c_int32),
         ("log_rays", ctypes.c_int32),
         ("log_bounces", ctypes.c_int32),
+        ("visual_limit", ctypes.c_int32),
+        ("visual_stride", ctypes.c_int32),
     ]
 
 
@@ -98,6 +100,12 @@ class TraceOutputs(ctypes.Structure):
         ("d_counters", ctypes.c_void_p),
         ("d_log_triangle", ctypes.c_void_p),
         ("d_log_t", ctypes.c_void_p),
+        ("d_visual_points", ctypes.c_void_p),
+        ("d_visual_surfaces", ctypes.c_void_p),
+        ("d_visual_rays", ctypes.c_void_p),
+        ("d_visual_orders", ctypes.c_void_p),
+        ("d_visual_distances", ctypes.c_void_p),
+        ("d_visual_gains", ctypes.c_void_p),
     ]
 
 

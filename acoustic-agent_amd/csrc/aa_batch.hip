@@ -161,7 +161,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
         lds.contrib = lds_surf ? my_contrib : nullptr;
         lds.senergy = lds_surf ? my_energy : nullptr;
         LaneStats st;
-        trace_rays_refill<WG, false>(sv, p, out, lds, &s_cursor[wave], begin, count, st, error_flag);
+        trace_rays_refill<WG, false, false>(sv, p, out, lds, &s_cursor[wave], begin, count, st, error_flag);
         flush_stats(st, p, out);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -275,6 +275,8 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
                                     t.d_diffuse_random, t.d_diffuse_indices, &t.outputs);
         if (rc != AA_OK) return rc;
         if (t.scene->device != b->device) return fail(AA_ERR_INVALID, "trace %d: scene on another device", i);
+        if (t.params.visual_limit > 0)
+            return fail(AA_ERR_INVALID, "trace %d: visual candidates are recorded by aa_trace only", i);
     }
     std::lock_guard<std::mutex> lock(b->mutex);
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);

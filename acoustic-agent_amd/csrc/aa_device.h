@@ -82,6 +82,14 @@ struct Outputs {
     int* log_tri;
     float* log_t;
     int log_rays, log_bounces;
+    // visual path candidates (null / 0 when off)
+    float* vis_points;  // [V,B,3]
+    int* vis_surf;      // [V,B]
+    int* vis_rays;      // [V]
+    int* vis_orders;    // [V]
+    float* vis_dist;    // [V]
+    float* vis_gain;    // [V]
+    int vis_limit, vis_stride, vis_bounces;
 };
 
 // Per-lane tallies flushed once per wave.
@@ -349,6 +357,16 @@ __device__ __forceinline__ void finish_ray(const RayParams& p, const Outputs& ou
 }
 
 // Start `ray`; returns false if it has no bounce to trace (num_bounces == 0).
+// Visual candidate slot of `ray`, or -1 (steam_rt.py:4724-4729): rays with
+// ray % stride == 0 while ray / stride is below the limit.
+__device__ __forceinline__ int visual_slot(const Outputs& out, int ray) {
+    if (out.vis_limit <= 0 || ray % out.vis_stride != 0) return -1;
+    const int v = ray / out.vis_stride;
+    return v < out.vis_limit ? v : -1;
+}
+
+// VISUAL: the caller may record visual candidates (the batch kernel never does).
+template <bool VISUAL = true>
 __device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out, int task, RayState& s,
                                           LaneStats& st) {
     const int group = task / p.num_rays;  // group-major: a wave's claims mostly share a group
@@ -368,6 +386,10 @@ __device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out
     for (int b = 0; b < kBands; ++b) s.A[b] = 1.0f;
     s.accum = 0.0f;
     s.mask = s.group_mask;
+    if (VISUAL && out.vis_limit > 0) {
+        const int v = visual_slot(out, ray);
+        if (v >= 0) out.vis_rays[v] = ray;
+    }
     if (p.num_bounces <= 0) {
         finish_ray(p, out, s, true, 0, st);
         return false;
@@ -636,8 +658,8 @@ __device__ __forceinline__ bool next_connection(const SceneView& sv, const RayPa
 
 // Deposit of an unoccluded connection (steam_rt.py:4810-4826).
 template <bool LDS_HIST>
-__device__ __forceinline__ void deposit(const RayParams& p, const Outputs& out, const Lds& lds, int num_surfaces,
-                                        const Connection& cn, LaneStats& st) {
+__device__ __forceinline__ float deposit(const RayParams& p, const Outputs& out, const Lds& lds, int num_surfaces,
+                                         const Connection& cn, LaneStats& st) {
     const int K = p.num_bins;
     const int k = cn.ks & 63, surf = cn.ks >> 6;
     float esum = 0.0f;
@@ -665,6 +687,23 @@ __device__ __forceinline__ void deposit(const RayParams& p, const Outputs& out, 
     } else {
         add_global(&out.contrib_counts[(size_t)k * num_surfaces + surf], 1);
         add_global(&out.surface_energy[(size_t)k * num_surfaces + surf], esum);
+    }
+    return esum;
+}
+
+// A visual candidate's strongest contribution so far (steam_rt.py:4821-4824:
+// replaced when the band energy sum exceeds the square of the stored root).
+// Candidates resolve their own connections, so only the ray's own lane
+// updates its slot, in bounce order.
+__device__ __forceinline__ void visual_update(const Outputs& out, const RayState& s, const HitContext& c,
+                                              const Query& q, float esum) {
+    const int v = visual_slot(out, s.ray);
+    if (v < 0) return;
+    const float g = out.vis_gain[v];
+    if (esum > g * g) {
+        out.vis_orders[v] = s.bounce + 1;
+        out.vis_dist[v] = s.accum + c.t + q.tmax;  // q.tmax is the distance to the source
+        out.vis_gain[v] = sqrtf(esum > 0.0f ? esum : 0.0f);
     }
 }
 
@@ -778,6 +817,7 @@ __device__ __forceinline__ int continue_ray(const SceneView& sv, const RayParams
 
 // A finished bounce query (steam_rt.py:4744-4769): termination tests, hit
 // counts and the hit context.  Returns false if the ray has ended.
+template <bool VISUAL = true>
 __device__ __forceinline__ bool close_bounce(const SceneView& sv, const RayParams& p, const Outputs& out,
                                             const Lds& lds, RayState& s, HitContext& c, const Query& q,
                                             LaneStats& st) {
@@ -827,6 +867,16 @@ __device__ __forceinline__ bool close_bounce(const SceneView& sv, const RayParam
     c.hx = s.ox + h.t * s.dx + kHitOffset * nx;
     c.hy = s.oy + h.t * s.dy + kHitOffset * ny;
     c.hz = s.oz + h.t * s.dz + kHitOffset * nz;
+    if (VISUAL && out.vis_limit > 0) {
+        const int v = visual_slot(out, s.ray);
+        if (v >= 0) {
+            float* vp = out.vis_points + ((size_t)v * out.vis_bounces + bounce) * 3;
+            vp[0] = c.hx;
+            vp[1] = c.hy;
+            vp[2] = c.hz;
+            out.vis_surf[(size_t)v * out.vis_bounces + bounce] = c.surf;
+        }
+    }
     unsigned long long m = s.mask;
     while (m) {
         const int k = __builtin_ctzll(m);
@@ -870,7 +920,7 @@ constexpr int kFinishEighths = 7;
 // `first`) with the calling wave's lanes.  A lane holds a ray (bounce queries,
 // and its own connections when the queue is full) or a parked connection of
 // another lane's ray; idle lanes take parked connections first, then new rays.
-template <int WG, bool LDS_HIST>
+template <int WG, bool LDS_HIST, bool VISUAL = true>
 __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds,
                                   unsigned* cursor, int first, int count, LaneStats& st, int* error_flag) {
     unsigned short* stk = lds.stack + threadIdx.x;
@@ -901,10 +951,13 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
             bool nee = false;  // the lane's ray has connections of its current hit to resolve
             if (phase != kIdle && !walking) {
                 if (phase == kBounce) {
-                    nee = close_bounce(sv, p, out, lds, s, c, q, st);
+                    nee = close_bounce<VISUAL>(sv, p, out, lds, s, c, q, st);
                     if (!nee) phase = kIdle;
                 } else {
-                    if (!q.occluded) deposit<LDS_HIST>(p, out, lds, sv.num_surfaces, cn, st);
+                    if (!q.occluded) {
+                        const float esum = deposit<LDS_HIST>(p, out, lds, sv.num_surfaces, cn, st);
+                        if (VISUAL && own && out.vis_limit > 0) visual_update(out, s, c, q, esum);
+                    }
                     nee = own;
                     if (!own) phase = kIdle;
                 }
@@ -920,11 +973,14 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
                         phase = continue_ray(sv, p, out, s, c, q, st);
                     }
                 }
-                const unsigned long long want = __ballot(has);
+                // a visual candidate resolves its own connections (its lane
+                // holds the ray state its slot update needs)
+                const bool self = VISUAL && has && out.vis_limit > 0 && visual_slot(out, s.ray) >= 0;
+                const unsigned long long want = __ballot(has && !self);
                 const int room = kQueueCap - q_count;
                 if (has) {
                     const int rank = __popcll(want & below);
-                    if (rank < room) {
+                    if (!self && rank < room) {
                         park(qu, (q_head + q_count + rank) & (kQueueCap - 1), q, cn);
                     } else {
                         phase = kShadow;
@@ -953,7 +1009,7 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
             if (!exhausted) {
                 const bool need = phase == kIdle;
                 const int got = claim_rays(need, cursor, count);
-                if (need && got >= 0 && start_ray(p, out, first + got, s, st)) {
+                if (need && got >= 0 && start_ray<VISUAL>(p, out, first + got, s, st)) {
                     begin_query(q, s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, kBig);
                     ++st.q_b;
                     phase = kBounce;
@@ -1079,6 +1135,16 @@ inline Outputs make_outputs(const aa_trace_outputs* o, const aa_trace_params* p)
     out.log_bounces = log ? p->log_bounces : 0;
     out.log_tri = log ? o->d_log_triangle : nullptr;
     out.log_t = log ? o->d_log_t : nullptr;
+    const bool vis = p->visual_limit > 0;
+    out.vis_limit = vis ? p->visual_limit : 0;
+    out.vis_stride = vis ? p->visual_stride : 1;
+    out.vis_bounces = p->num_bounces;
+    out.vis_points = vis ? o->d_visual_points : nullptr;
+    out.vis_surf = vis ? o->d_visual_surfaces : nullptr;
+    out.vis_rays = vis ? o->d_visual_rays : nullptr;
+    out.vis_orders = vis ? o->d_visual_orders : nullptr;
+    out.vis_dist = vis ? o->d_visual_distances : nullptr;
+    out.vis_gain = vis ? o->d_visual_gains : nullptr;
     return out;
 }
 

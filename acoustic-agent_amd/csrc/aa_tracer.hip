@@ -72,6 +72,11 @@ __global__ void fill_zero_kernel(unsigned* p, size_t words) {
         p[i] = 0u;
 }
 
+__global__ void fill_word_kernel(unsigned* p, size_t words, unsigned value) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
 // Persistent lanes over the rays of one trace (refilled as rays end).  The
 // register budget is held to 3 waves per SIMD (168 VGPRs, no spills): the
 // tracer needs waves in flight to hide its dependent loads.
@@ -649,6 +654,16 @@ int aa::validate_trace(const aa_scene_s* scene, const aa_trace_params* p, const 
         return fail(AA_ERR_INVALID, "hit log needs d_log_triangle and d_log_t");
     if ((int64_t)p->num_sources * AA_NUM_BANDS * p->num_bins > (int64_t)1 << 30)
         return fail(AA_ERR_LIMIT, "histogram too large");
+    if (p->visual_limit < 0) return fail(AA_ERR_INVALID, "visual_limit must be >= 0");
+    if (p->visual_limit > 0) {
+        if (p->num_sources != 1) return fail(AA_ERR_INVALID, "visual candidates need a single-source trace");
+        if (p->visual_stride < 1) return fail(AA_ERR_INVALID, "visual_stride must be >= 1");
+        if (!o->d_visual_points || !o->d_visual_surfaces || !o->d_visual_rays || !o->d_visual_orders ||
+            !o->d_visual_distances || !o->d_visual_gains)
+            return fail(AA_ERR_INVALID, "visual_limit > 0 needs every d_visual_* output");
+        if ((int64_t)p->visual_limit * (p->num_bounces > 0 ? p->num_bounces : 1) * 3 > (int64_t)1 << 30)
+            return fail(AA_ERR_LIMIT, "visual candidate buffers too large");
+    }
     return AA_OK;
 }
 
@@ -660,6 +675,28 @@ hipError_t zero_range(void* ptr, size_t bytes, hipStream_t stream) {
     const int grid = (int)std::min<size_t>((words + 255) / 256, 1024);
     hipLaunchKernelGGL(aa::fill_zero_kernel, dim3(grid), dim3(256), 0, stream, static_cast<unsigned*>(ptr), words);
     return hipGetLastError();
+}
+
+hipError_t fill_words(void* ptr, size_t words, unsigned value, hipStream_t stream) {
+    if (!ptr || words == 0) return hipSuccess;
+    const int grid = (int)std::min<size_t>((words + 255) / 256, 1024);
+    hipLaunchKernelGGL(aa::fill_word_kernel, dim3(grid), dim3(256), 0, stream, static_cast<unsigned*>(ptr), words,
+                       value);
+    return hipGetLastError();
+}
+
+// Initial values of the visual candidate outputs: no hit (surface / ray -1),
+// no contribution (order, distance, gain 0), points at the origin.
+hipError_t init_visual(const aa_trace_outputs* o, const aa_trace_params* p, hipStream_t stream) {
+    if (p->visual_limit <= 0) return hipSuccess;
+    const size_t V = (size_t)p->visual_limit, B = (size_t)(p->num_bounces > 0 ? p->num_bounces : 0);
+    hipError_t e;
+    if ((e = zero_range(o->d_visual_points, V * B * 3 * 4, stream)) != hipSuccess) return e;
+    if ((e = fill_words(o->d_visual_surfaces, V * B, 0xffffffffu, stream)) != hipSuccess) return e;
+    if ((e = fill_words(o->d_visual_rays, V, 0xffffffffu, stream)) != hipSuccess) return e;
+    if ((e = zero_range(o->d_visual_orders, V * 4, stream)) != hipSuccess) return e;
+    if ((e = zero_range(o->d_visual_distances, V * 4, stream)) != hipSuccess) return e;
+    return zero_range(o->d_visual_gains, V * 4, stream);
 }
 
 }  // namespace
@@ -718,6 +755,7 @@ extern "C" int aa_trace(aa_scene_t scene, const aa_trace_params* p, const float*
     // zero the outputs with a kernel (capturable, unlike hipMemsetAsync)
     const aa::OutputExtents ext = aa::output_extents(o, p, S);
     for (int i = 0; i < 8; ++i) AA_HIP(zero_range(ext.ptr[i], ext.bytes[i], stream));
+    AA_HIP(init_visual(o, p, stream));
 
     hipError_t e = hipSuccess;
 #define AA_LAUNCH(W)                                                                   \
@@ -742,6 +780,7 @@ extern "C" int aa_trace_plain(aa_scene_t scene, const aa_trace_params* p, const 
     int rc = aa::validate_trace(scene, p, d_sources, d_directions, d_diffuse_bank, d_diffuse_random,
                                 d_diffuse_indices, o);
     if (rc != AA_OK) return rc;
+    if (p->visual_limit > 0) return fail(AA_ERR_INVALID, "visual candidates are recorded by aa_trace only");
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
     AA_HIP(hipSetDevice(scene->device));
     aa::PlainArgs a;

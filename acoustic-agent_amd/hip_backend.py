@@ -252,13 +252,19 @@ class TraceBuffers:
     counters: Any
     log_triangle: Any
     log_t: Any
+    visual: tuple | None = None  # points, surfaces, rays, orders, distances, gains (visual candidates)
 
     @classmethod
     def allocate(cls, device: int, num_sources: int, num_bins: int, num_surfaces: int, ambisonic: bool,
-                 log_rays: int = 0, log_bounces: int = 0) -> "TraceBuffers":
+                 log_rays: int = 0, log_bounces: int = 0, visual_limit: int = 0,
+                 num_bounces: int = 0) -> "TraceBuffers":
         torch = _torch()
         dev = f"cuda:{device}"
         f32, i32 = torch.float32, torch.int32
+        V, B = int(visual_limit), max(int(num_bounces), 0)
+        visual = (torch.empty((V, B, 3), dtype=f32, device=dev), torch.empty((V, B), dtype=i32, device=dev),
+                  torch.empty((V,), dtype=i32, device=dev), torch.empty((V,), dtype=i32, device=dev),
+                  torch.empty((V,), dtype=f32, device=dev), torch.empty((V,), dtype=f32, device=dev)) if V > 0 else None
         return cls(
             echogram=torch.empty((num_sources, 6, num_bins), dtype=f32, device=dev),
             ambisonic=torch.empty((num_sources, 6, 4, num_bins), dtype=f32, device=dev) if ambisonic else None,
@@ -271,15 +277,17 @@ class TraceBuffers:
             # bounces a ray never reaches keep -2 / 0
             log_triangle=torch.full((log_rays, log_bounces), -2, dtype=i32, device=dev) if log_rays > 0 else None,
             log_t=torch.zeros((log_rays, log_bounces), dtype=f32, device=dev) if log_rays > 0 else None,
+            visual=visual,
         )
 
     def as_struct(self) -> _lib.TraceOutputs:
         def p(t):
             return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
+        vis = [p(t) for t in self.visual] if self.visual is not None else [ctypes.c_void_p(0)] * 6
         return _lib.TraceOutputs(p(self.echogram), p(self.ambisonic), p(self.hit_counts), p(self.contrib_counts),
                                  p(self.surface_energy), p(self.actual_bounces), p(self.active_count), p(self.counters),
-                                 p(self.log_triangle), p(self.log_t))
+                                 p(self.log_triangle), p(self.log_t), *vis)
 
 
 def make_params(*, num_rays: int, num_bounces: int, num_bins: int, num_diffuse_samples: int, num_sources: int,
@@ -287,7 +295,8 @@ def make_params(*, num_rays: int, num_bounces: int, num_bins: int, num_diffuse_s
                 bin_dur: float, speed_of_sound: float, max_path_len: float, listener_radius: float,
                 source_radius: float, irradiance_min_distance: float, specular_exponent: float,
                 dipole_weight: float, dipole_power: float, log_rays: int = 0, log_bounces: int = 0,
-                tuning: Mapping[str, int] | None = None) -> _lib.TraceParams:
+                tuning: Mapping[str, int] | None = None, visual_limit: int = 0,
+                visual_stride: int = 1) -> _lib.TraceParams:
     t = dict(tuning or {})
     p = _lib.TraceParams()
     p.num_rays = int(num_rays)
@@ -318,6 +327,8 @@ def make_params(*, num_rays: int, num_bounces: int, num_bins: int, num_diffuse_s
     p.sources_per_lane = int(t.get("sources_per_lane", 0))
     p.log_rays = int(log_rays)
     p.log_bounces = int(log_bounces)
+    p.visual_limit = max(int(visual_limit), 0)
+    p.visual_stride = max(int(visual_stride), 1)
     return p
 
 
@@ -340,6 +351,19 @@ def launch_trace(scene: DeviceScene, params: _lib.TraceParams, sources_dev, bank
     fn = lib.aa_trace_plain if plain else lib.aa_trace
     _lib.check(fn(scene.handle, ctypes.byref(params), p(sources_dev), p(directions_dev), p(bank_dev),
                   p(random_dev), p(indices_dev), ctypes.byref(out), ctypes.c_void_p(s.cuda_stream)))
+
+
+_VISUAL_KEYS = ("visual_hit_points", "visual_surface_indices", "visual_ray_indices", "visual_orders",
+                "visual_distances", "visual_gains")
+
+
+def _visual_result(buffers: TraceBuffers, num_bounces: int) -> dict[str, np.ndarray]:
+    """The raw visual keys of steam_rt_cuda.py:209-214 (empty when not requested)."""
+    if buffers.visual is None:
+        empty = (np.zeros((0, num_bounces, 3), np.float32), np.zeros((0, num_bounces), np.int32),
+                 np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.float32), np.zeros(0, np.float32))
+        return dict(zip(_VISUAL_KEYS, empty))
+    return {k: t.cpu().numpy() for k, t in zip(_VISUAL_KEYS, buffers.visual)}
 
 
 def trace_energy_field_hip(
@@ -429,8 +453,11 @@ def trace_energy_field_hip(
         dirs_dev = _device_input(directions, np.float32, device, cache_stats)
         rand_dev = _device_input(diffuse_random, np.float32, device, cache_stats)
         idx_dev = _device_input(diffuse_indices, np.int32, device, cache_stats)
+    V = max(int(visual_candidate_limit), 0)
+    if V > 0 and (sources is not None and Ks != 1):
+        raise ValueError("visual candidates need a single-source trace")
     buffers = TraceBuffers.allocate(device, Ks, int(num_bins), scene.num_surfaces, bool(render_ambisonics),
-                                    int(log_rays), int(log_bounces))
+                                    int(log_rays), int(log_bounces), visual_limit=V, num_bounces=int(num_bounces))
     params = make_params(
         num_rays=n_rays, num_bounces=num_bounces, num_bins=num_bins,
         num_diffuse_samples=int(np.asarray(diffuse_bank).shape[0]), num_sources=Ks,
@@ -439,7 +466,8 @@ def trace_energy_field_hip(
         bin_dur=bin_dur, speed_of_sound=speed_of_sound, max_path_len=max_path_len, listener_radius=listener_radius,
         source_radius=source_radius, irradiance_min_distance=irradiance_min_distance,
         specular_exponent=specular_exponent, dipole_weight=dipole_weight, dipole_power=dipole_power,
-        log_rays=min(int(log_rays), n_rays), log_bounces=int(log_bounces), tuning=tuning)
+        log_rays=min(int(log_rays), n_rays), log_bounces=int(log_bounces), tuning=tuning, visual_limit=V,
+        visual_stride=visual_stride)
     stream = torch.cuda.current_stream(device)
     stream.synchronize()
     upload_time = time.perf_counter() - started
@@ -465,12 +493,7 @@ def trace_energy_field_hip(
         "actual_bounces": (int(buffers.actual_bounces.cpu()[0]) if sources is None
                            else buffers.actual_bounces.cpu().numpy()),
         "active_count": int(buffers.active_count.cpu()[0]) if sources is None else buffers.active_count.cpu().numpy(),
-        "visual_hit_points": np.zeros((0, int(num_bounces), 3), dtype=np.float32),
-        "visual_surface_indices": np.zeros((0, int(num_bounces)), dtype=np.int32),
-        "visual_ray_indices": np.zeros(0, dtype=np.int32),
-        "visual_orders": np.zeros(0, dtype=np.int32),
-        "visual_distances": np.zeros(0, dtype=np.float32),
-        "visual_gains": np.zeros(0, dtype=np.float32),
+        **_visual_result(buffers, int(num_bounces)),
         "counters": dict(zip(_lib.COUNTER_NAMES, (int(x) for x in buffers.counters.cpu().numpy()))),
     }
     if buffers.log_triangle is not None:

@@ -38,6 +38,8 @@ class SimConfig:
     intersection_backend: str = "auto"
     bvh_min_surfaces: int = 16
     collect_visual_paths: bool = False
+    rt_visual_num_rays: int | None = None
+    rt_visual_num_bounces: int | None = None
     render_ambisonics: bool = True
     # HIP backend
     rt_hip_device: int | None = None   # None: use rt_cuda_device

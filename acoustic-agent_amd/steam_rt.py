@@ -37,6 +37,17 @@ AIR_ABSORPTION_NP_PER_M = {
     "4000": 0.0034539,
 }
 _SH_Y00 = 1.0 / math.sqrt(4.0 * math.pi)
+_RT_VISUAL_RETAIN_LIMIT = 512  # steam_rt.py:41
+
+
+def visual_candidate_plan(config: SimConfig, num_rays: int) -> tuple[int, int]:
+    """(limit, stride) of the visual path candidates (steam_rt.py:1853-1860):
+    up to 4 x 512 (or rt_visual_num_rays) rays spread evenly over the ray
+    set, none unless collect_visual_paths."""
+    default = int(_RT_VISUAL_RETAIN_LIMIT * 4)
+    requested = default if config.rt_visual_num_rays is None else max(1, min(int(config.rt_visual_num_rays), default))
+    limit = min(int(num_rays), requested) if config.collect_visual_paths else 0
+    return limit, max(1, int(num_rays) // max(limit, 1))
 
 
 def _normalize_rt_accelerator(value: str) -> str:
@@ -121,6 +132,7 @@ def _trace_energy_field_hip(
     else:
         raise ValueError("rt_rng must be 'counter' or 'reference'")
     direct_delay = float(np.linalg.norm(np.asarray(source, float) - np.asarray(listener, float))) / float(config.c)
+    visual_limit, visual_stride = visual_candidate_plan(config, num_rays)
     raw = trace_energy_field_hip(
         source=np.asarray(source, dtype=np.float32),
         listener=np.asarray(listener, dtype=np.float32),
@@ -149,14 +161,19 @@ def _trace_energy_field_hip(
         seed=int(config.seed),
         num_rays=num_rays,
         tessellation_max_edge=config.rt_tessellation_max_edge_m,
+        visual_candidate_limit=visual_limit,
+        visual_stride=visual_stride,
     )
     return field_from_raw(raw, arrays["names"], num_bins, bin_dur, direct_delay, render_ambisonics,
-                          intersection_backend, emitter)
+                          intersection_backend, emitter,
+                          visual=(visual_limit, visual_stride) if config.collect_visual_paths else None)
 
 
 def field_from_raw(raw: Mapping[str, Any], names, num_bins: int, bin_dur: float, direct_delay: float,
-                   render_ambisonics: bool, intersection_backend: str, emitter: Mapping[str, Any]) -> dict[str, Any]:
-    """The field dictionary of steam_rt.py:1889-1923 from one raw trace."""
+                   render_ambisonics: bool, intersection_backend: str, emitter: Mapping[str, Any],
+                   visual: tuple[int, int] | None = None) -> dict[str, Any]:
+    """The field dictionary of steam_rt.py:1889-1923 from one raw trace;
+    ``visual`` = (candidate limit, stride) when visual paths were collected."""
     echogram = np.asarray(raw["echogram"])
     total_energy = np.sum(echogram, axis=0)
     nonzero = np.flatnonzero(total_energy > 0.0)
@@ -173,7 +190,17 @@ def field_from_raw(raw: Mapping[str, Any], names, num_bins: int, bin_dur: float,
         "surface_hit_count": {names[i]: int(hits[i]) for i in range(len(names)) if int(hits[i]) > 0},
         "surface_contribution_count": {names[i]: int(contribs[i]) for i in range(len(names)) if int(contribs[i]) > 0},
         "surface_energy": {names[i]: float(energy[i]) for i in range(len(names)) if float(energy[i]) > 0.0},
-        "visual_candidates": None,
+        "visual_candidates": {
+            "hit_points": raw["visual_hit_points"],
+            "surface_indices": raw["visual_surface_indices"],
+            "ray_indices": raw["visual_ray_indices"],
+            "orders": raw["visual_orders"],
+            "distances": raw["visual_distances"],
+            "gains": raw["visual_gains"],
+            "surface_names": tuple(names),
+            "candidate_limit": int(visual[0]),
+            "stride": int(visual[1]),
+        } if visual is not None else None,
         "accelerator": "hip",
         "precision": "float32",
         "hip": raw.get("device"),

@@ -145,6 +145,11 @@ typedef struct {
                                     (0: 8); the trace runs num_rays x ceil(Ks / this) lane tasks */
     int32_t log_rays;            /* hit-log extent (0: off) */
     int32_t log_bounces;
+    int32_t visual_limit;        /* visual path candidates (steam_rt.py:4718-4731, 4765-4769,
+                                    4821-4824): ray r with r % visual_stride == 0 and
+                                    r / visual_stride < visual_limit records its hit points and its
+                                    strongest contribution; 0: off.  aa_trace with Ks = 1 only. */
+    int32_t visual_stride;
 } aa_trace_params;
 
 typedef struct {
@@ -158,6 +163,12 @@ typedef struct {
     int64_t *d_counters;      /* [AA_CNT_COUNT] */
     int32_t *d_log_triangle;  /* [log_rays, log_bounces] hit triangle (-1 none) or NULL */
     float *d_log_t;           /* [log_rays, log_bounces] or NULL */
+    float *d_visual_points;   /* [V,B,3] offset hit point per bounce (0: none); V = visual_limit */
+    int32_t *d_visual_surfaces;   /* [V,B] surface id per bounce (-1: none) */
+    int32_t *d_visual_rays;       /* [V] ray index of the candidate (-1: none) */
+    int32_t *d_visual_orders;     /* [V] bounce + 1 of the strongest contribution (0: none) */
+    float *d_visual_distances;    /* [V] its path length */
+    float *d_visual_gains;        /* [V] sqrt of its band energy sum */
 } aa_trace_outputs;
 
 /* d_sources: [Ks,4] device floats (x, y, z, direct_delay_s).

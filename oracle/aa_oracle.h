@@ -56,6 +56,7 @@
         const int32_t *diffuse_indices;  /* [B,N]  (rng_mode 0) */                      \
         int32_t log_rays, log_bounces;                                                  \
         int32_t ray_begin, ray_end;      /* traced ray range; ray_end 0 = all */        \
+        int32_t visual_limit, visual_stride; /* candidate rays ri % stride == 0 */      \
     } ao_params_##SUF;                                                                  \
                                                                                         \
     typedef struct {                                                                    \
@@ -67,6 +68,12 @@
         int32_t *log_prim;      /* [log_rays, log_bounces] or NULL */                   \
         int32_t *log_surface;                                                           \
         double *log_t;                                                                  \
+        REALT *visual_points;      /* [V,B,3] offset hit points, or NULL */             \
+        int32_t *visual_surfaces;  /* [V,B] */                                          \
+        int32_t *visual_rays;      /* [V] */                                            \
+        int32_t *visual_orders;    /* [V] bounce + 1 of the strongest contribution */   \
+        REALT *visual_distances;   /* [V] its path length */                            \
+        REALT *visual_gains;       /* [V] sqrt of its band energy sum */                \
         int64_t actual_bounces, active_count;                                           \
         int64_t bounce_queries, shadow_queries;                                         \
         int64_t nodes_bounce, nodes_shadow, prims_bounce, prims_shadow;                 \

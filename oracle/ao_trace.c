@@ -429,6 +429,23 @@ int CAT(ao_trace, SUF)(const SCENE *sc, const PARAMS *p, OUTS *out)
     const REAL source_distance =
         RSQRT((L[0] - Sp[0]) * (L[0] - Sp[0]) + (L[1] - Sp[1]) * (L[1] - Sp[1]) + (L[2] - Sp[2]) * (L[2] - Sp[2]));
 
+    /* visual candidates (steam_rt.py:4718-4731): ray ri % stride == 0 owns
+       slot ri / stride while that is below the limit */
+    const int V = (out->visual_points && p->visual_limit > 0) ? p->visual_limit : 0;
+    const int vstride = p->visual_stride > 0 ? p->visual_stride : 1;
+    for (int v = 0; v < V; ++v) {
+        for (int b = 0; b < B; ++b) {
+            out->visual_points[((size_t)v * B + b) * 3] = 0;
+            out->visual_points[((size_t)v * B + b) * 3 + 1] = 0;
+            out->visual_points[((size_t)v * B + b) * 3 + 2] = 0;
+            out->visual_surfaces[(size_t)v * B + b] = -1;
+        }
+        out->visual_rays[v] = -1;
+        out->visual_orders[v] = 0;
+        out->visual_distances[v] = 0;
+        out->visual_gains[v] = 0;
+    }
+
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static) num_threads(nthreads)
 #endif
@@ -457,6 +474,8 @@ int CAT(ao_trace, SUF)(const SCENE *sc, const PARAMS *p, OUTS *out)
         for (int b = 0; b < NB; ++b) accum_energy[b] = 1;
         int alive = 1;
         const int logging = out->log_prim && ri < p->log_rays;
+        const int vslot = (V > 0 && ri % vstride == 0 && ri / vstride < V) ? ri / vstride : -1;
+        if (vslot >= 0) out->visual_rays[vslot] = ri;
         for (int bounce = 0; bounce < B; ++bounce) {
             if (bounce + 1 > lc->actual_bounces) lc->actual_bounces = bounce + 1;
             int prim, surf;
@@ -488,6 +507,13 @@ int CAT(ao_trace, SUF)(const SCENE *sc, const PARAMS *p, OUTS *out)
             REAL hx = origin[0] + t * dir[0] + HIT_OFFSET * n[0];
             REAL hy = origin[1] + t * dir[1] + HIT_OFFSET * n[1];
             REAL hz = origin[2] + t * dir[2] + HIT_OFFSET * n[2];
+            if (vslot >= 0) {
+                REAL *vp = out->visual_points + ((size_t)vslot * B + bounce) * 3;
+                vp[0] = hx;
+                vp[1] = hy;
+                vp[2] = hz;
+                out->visual_surfaces[(size_t)vslot * B + bounce] = surf;
+            }
             REAL tsx = Sp[0] - hx, tsy = Sp[1] - hy, tsz = Sp[2] - hz;
             REAL dist_to_source = RSQRT(tsx * tsx + tsy * tsy + tsz * tsz);
             if (dist_to_source > p->irradiance_min_distance) {
@@ -538,6 +564,12 @@ int CAT(ao_trace, SUF)(const SCENE *sc, const PARAMS *p, OUTS *out)
                                     a[3 * K + bin] += energy * dir0[2];
                                 }
                                 energy_sum += energy;
+                            }
+                            /* the strongest contribution (steam_rt.py:4821-4824) */
+                            if (vslot >= 0 && energy_sum > out->visual_gains[vslot] * out->visual_gains[vslot]) {
+                                out->visual_orders[vslot] = bounce + 1;
+                                out->visual_distances[vslot] = accum_distance + t + dist_to_source;
+                                out->visual_gains[vslot] = RSQRT(energy_sum > 0 ? energy_sum : 0);
                             }
                             lc->contrib_counts[surf] += 1;
                             lc->surface_energy[surf] += energy_sum;

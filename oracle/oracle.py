@@ -68,12 +68,15 @@ def _structs(real):
             ("specular_exponent", real), ("dipole_weight", real), ("dipole_power", real),
             ("directions", _P), ("diffuse_bank", _P), ("diffuse_random", _P), ("diffuse_indices", _P),
             ("log_rays", _I32), ("log_bounces", _I32), ("ray_begin", _I32), ("ray_end", _I32),
+            ("visual_limit", _I32), ("visual_stride", _I32),
         ]
 
     class Out(ctypes.Structure):
         _fields_ = [
             ("echogram", _P), ("ambisonic", _P), ("hit_counts", _P), ("contrib_counts", _P),
             ("surface_energy", _P), ("log_prim", _P), ("log_surface", _P), ("log_t", _P),
+            ("visual_points", _P), ("visual_surfaces", _P), ("visual_rays", _P), ("visual_orders", _P),
+            ("visual_distances", _P), ("visual_gains", _P),
             ("actual_bounces", _I64), ("active_count", _I64), ("bounce_queries", _I64), ("shadow_queries", _I64),
             ("nodes_bounce", _I64), ("nodes_shadow", _I64), ("prims_bounce", _I64), ("prims_shadow", _I64),
             ("contributions", _I64), ("escaped", _I64),
@@ -231,7 +234,8 @@ def trace(scene: OracleScene, *, source, listener, num_rays: int, num_bounces: i
           dipole_power: float = 1.0, diffuse_bank: np.ndarray, rng: str = "reference", directions=None,
           diffuse_random=None, diffuse_indices=None, seed: int = 0, render_ambisonics: bool = False,
           shadow_closest: bool = False, threads: int = 0, log_rays: int = 0, log_bounces: int = 0,
-          ray_range: tuple[int, int] | None = None) -> dict[str, Any]:
+          ray_range: tuple[int, int] | None = None, visual_limit: int = 0,
+          visual_stride: int = 1) -> dict[str, Any]:
     prec = scene.precision
     real = np.float64 if prec == "float64" else np.float32
     _, Params, Out = _STRUCTS[prec]
@@ -288,6 +292,20 @@ def trace(scene: OracleScene, *, source, listener, num_rays: int, num_bounces: i
     out.hit_counts, out.contrib_counts, out.surface_energy = _p(hits), _p(contribs), _p(senergy)
     if log_rays > 0 and log_bounces > 0:
         out.log_prim, out.log_surface, out.log_t = _p(log_prim), _p(log_surf), _p(log_t)
+    V = max(int(visual_limit), 0)
+    p.visual_limit, p.visual_stride = V, max(int(visual_stride), 1)
+    vis = {
+        "visual_hit_points": np.zeros((V, num_bounces, 3), dtype=real),
+        "visual_surface_indices": np.full((V, num_bounces), -1, dtype=np.int32),
+        "visual_ray_indices": np.full(V, -1, dtype=np.int32),
+        "visual_orders": np.zeros(V, dtype=np.int32),
+        "visual_distances": np.zeros(V, dtype=real),
+        "visual_gains": np.zeros(V, dtype=real),
+    }
+    if V > 0:
+        out.visual_points, out.visual_surfaces = _p(vis["visual_hit_points"]), _p(vis["visual_surface_indices"])
+        out.visual_rays, out.visual_orders = _p(vis["visual_ray_indices"]), _p(vis["visual_orders"])
+        out.visual_distances, out.visual_gains = _p(vis["visual_distances"]), _p(vis["visual_gains"])
     fn = getattr(load(), "ao_trace_" + ("f64" if prec == "float64" else "f32"))
     rc = fn(ctypes.byref(scene.struct), ctypes.byref(p), ctypes.byref(out))
     if rc != 0:
@@ -306,4 +324,6 @@ def trace(scene: OracleScene, *, source, listener, num_rays: int, num_bounces: i
     }
     if log_rays > 0 and log_bounces > 0:
         res.update(log_prim=log_prim, log_surface=log_surf, log_t=log_t)
+    if V > 0:
+        res.update(vis)
     return res

@@ -48,7 +48,7 @@ class Case:
 
 
 def run_gpu(case: Case, *, use_bvh: bool = True, tuning=None, log_rays: int = 0, scene=None, source=None,
-            plain: bool = False):
+            plain: bool = False, visual_limit: int = 0, visual_stride: int = 1):
     from acoustic_agent_amd import hip_backend
 
     dirs, bank, U, J = case.ray_set()
@@ -67,11 +67,13 @@ def run_gpu(case: Case, *, use_bvh: bool = True, tuning=None, log_rays: int = 0,
         dipole_power=float(case.extra.get("dipole_power", 1.0)),
         render_ambisonics=case.ambisonics, device_id=0, rng_mode=case.rng, seed=case.seed,
         num_rays=case.num_rays, scene=scene, tessellation_max_edge=w.max_edge, log_rays=log_rays,
-        log_bounces=case.log_bounces if log_rays else 0, tuning=tuning, plain=plain, **kw)
+        log_bounces=case.log_bounces if log_rays else 0, tuning=tuning, plain=plain,
+        visual_candidate_limit=visual_limit, visual_stride=visual_stride, **kw)
 
 
 def run_oracle(case: Case, precision: str = "float32", *, triangles: bool = True, log_rays: int = 0,
-               source=None, threads: int = 0, use_bvh: bool = True, ray_range=None):
+               source=None, threads: int = 0, use_bvh: bool = True, ray_range=None, visual_limit: int = 0,
+               visual_stride: int = 1):
     dirs, bank, U, J = case.ray_set()
     w = case.workload
     src = w.source if source is None else np.asarray(source)
@@ -84,7 +86,7 @@ def run_oracle(case: Case, precision: str = "float32", *, triangles: bool = True
                    dipole_power=float(case.extra.get("dipole_power", 1.0)),
                    diffuse_random=U, diffuse_indices=J, seed=case.seed, render_ambisonics=case.ambisonics,
                    log_rays=log_rays, log_bounces=case.log_bounces if log_rays else 0, threads=threads,
-                   ray_range=ray_range)
+                   ray_range=ray_range, visual_limit=visual_limit, visual_stride=visual_stride)
 
 
 def rel_l1(a, b) -> float:

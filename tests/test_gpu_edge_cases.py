@@ -117,3 +117,45 @@ def test_scene_file_loads_as_the_same_device_scene(shoebox, tmp_path):
     assert np.array_equal(base["hit_counts"], again["hit_counts"])
     assert rel_l1(again["echogram"], base["echogram"]) <= 1e-5
     loaded.close()
+
+
+@pytest.mark.parametrize("which", ["shoebox", "lplan"])
+def test_visual_candidates_match_fp32_oracle(shoebox, which):
+    # the candidate rays' hit points, surfaces and strongest contributions
+    # (steam_rt.py:4718-4731, 4765-4769, 4821-4824) equal the binary32 oracle's
+    w = shoebox if which == "shoebox" else scenes.l_plan_workload(0, target_triangles=20000)
+    case = Case(w, num_rays=4096, num_bounces=60, num_bins=2000, seed=12)
+    V, stride = 256, 16
+    gpu = run_gpu(case, visual_limit=V, visual_stride=stride)
+    ref = run_oracle(case, "float32", visual_limit=V, visual_stride=stride)
+    np.testing.assert_array_equal(gpu["visual_ray_indices"], ref["visual_ray_indices"])
+    np.testing.assert_array_equal(gpu["visual_surface_indices"], ref["visual_surface_indices"])
+    np.testing.assert_array_equal(gpu["visual_orders"], ref["visual_orders"])
+    np.testing.assert_allclose(gpu["visual_hit_points"], ref["visual_hit_points"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(gpu["visual_distances"], ref["visual_distances"], rtol=1e-6)
+    np.testing.assert_allclose(gpu["visual_gains"], ref["visual_gains"], rtol=1e-5)
+    assert np.count_nonzero(gpu["visual_orders"]) > V // 4
+    # the rest of the trace is unchanged by recording candidates
+    plain = run_gpu(case)
+    np.testing.assert_array_equal(plain["hit_counts"], gpu["hit_counts"])
+    assert rel_l1(gpu["echogram"], plain["echogram"]) <= 1e-5
+
+
+def test_visual_candidates_in_the_field_and_rejected_elsewhere(shoebox):
+    from dataclasses import replace
+
+    from acoustic_agent_amd.models import SimConfig
+    from acoustic_agent_amd.steam_rt import trace_energy_field
+
+    cfg = SimConfig(rt_num_rays=3000, rt_num_bounces=20, duration_s=0.5, rt_duration_s=0.5, collect_visual_paths=True,
+                    rt_visual_num_rays=100)
+    field = trace_energy_field(shoebox.scene, shoebox.source, shoebox.listener, cfg)
+    vc = field["visual_candidates"]
+    assert vc["candidate_limit"] == 100 and vc["stride"] == 30
+    np.testing.assert_array_equal(vc["ray_indices"], np.arange(100) * 30)
+    assert vc["hit_points"].shape == (100, 20, 3) and np.count_nonzero(vc["orders"]) > 10
+    assert trace_energy_field(shoebox.scene, shoebox.source, shoebox.listener,
+                              replace(cfg, collect_visual_paths=False))["visual_candidates"] is None
+    case = Case(shoebox, num_rays=512, num_bounces=10, num_bins=500, seed=1)
+    with pytest.raises(ValueError, match="aa_trace only"):
+        run_gpu(case, visual_limit=8, visual_stride=4, plain=True)

@@ -491,3 +491,53 @@ def test_sah_tree_gives_the_same_hits_with_fewer_visits(precision):
     np.testing.assert_array_equal(ra["hit_counts"], rb["hit_counts"])
     np.testing.assert_array_equal(ra["echogram"], rb["echogram"])
     assert ra["counters"]["nodes_bounce"] < rb["counters"]["nodes_bounce"]
+
+
+def test_visual_candidates_follow_the_trajectories():
+    # rays r = v * stride record their offset hit points and surfaces: the
+    # surfaces are the hit log's while the ray lives (then -1), and each point
+    # lies t (+- the 0.01 hit offset) from the previous one, the listener first
+    w = scenes.shoebox_workload(0)
+    case = Case(w, num_rays=600, num_bounces=30, num_bins=1000, seed=4, log_bounces=30)
+    V, stride = 40, 15
+    ref = run_oracle(case, "float64", log_rays=600, visual_limit=V, visual_stride=stride)
+    np.testing.assert_array_equal(ref["visual_ray_indices"], np.arange(V) * stride)
+    for v in range(V):
+        r = v * stride
+        surf = ref["visual_surface_indices"][v]
+        alive = np.flatnonzero(surf >= 0)
+        n = alive.size
+        assert np.array_equal(alive, np.arange(n)) and np.all(surf[n:] == -1)
+        np.testing.assert_array_equal(surf[:n], ref["log_surface"][r, :n])
+        pts = ref["visual_hit_points"][v]
+        prev = np.asarray(w.listener, dtype=np.float64)
+        for b in range(n):
+            step = np.linalg.norm(pts[b] - prev)
+            assert abs(step - ref["log_t"][r, b]) <= 0.0101, (v, b, step, ref["log_t"][r, b])
+            prev = pts[b]
+        assert np.all(pts[n:] == 0)
+
+
+def test_visual_strongest_contribution_is_the_largest_bin_of_the_ray():
+    # one ray traced alone: each contribution adds its band energy sum to one
+    # bin, so (with distinct bins) the strongest one is the largest band-summed
+    # bin, and its path length maps back to that bin
+    w = scenes.l_plan_workload(0, target_triangles=3000)
+    case = Case(w, num_rays=256, num_bounces=40, num_bins=2000, seed=6)
+    checked = 0
+    for r in range(0, 256, 16):
+        one = run_oracle(case, "float64", ray_range=(r, r + 1), visual_limit=256, visual_stride=1)
+        order, gain, dist = one["visual_orders"][r], one["visual_gains"][r], one["visual_distances"][r]
+        per_bin = one["echogram"].sum(axis=0)
+        if order == 0:
+            assert gain == 0 and not np.any(per_bin > 0)
+            continue
+        occupied = np.count_nonzero(per_bin > 0)
+        if occupied != one["counters"]["contributions"]:
+            continue  # two contributions share a bin
+        j = int(np.argmax(per_bin))
+        assert gain ** 2 == pytest.approx(per_bin[j], rel=1e-12)
+        assert int(math.floor((dist / 343.0 - case.direct_delay()) / case.bin_dur)) == j
+        assert 1 <= order <= 40
+        checked += 1
+    assert checked >= 8
