@@ -154,6 +154,7 @@ _SIGNATURES = {
     "aa_trace_plain": (_I32, [_P, ctypes.POINTER(TraceParams), _P, _P, _P, _P, _P, ctypes.POINTER(TraceOutputs),
                               _P]),
     "aa_stream_sync": (_I32, [_P]),
+    "aa_stream_query": (_I32, [_P, ctypes.POINTER(_I32)]),
     "aa_batch_create": (_I32, [_I32, _I32, ctypes.POINTER(_P)]),
     "aa_batch_submit": (_I32, [_P, ctypes.POINTER(BatchTrace), _I32, ctypes.POINTER(BatchTuning), _P]),
     "aa_batch_release": (_I32, [_P]),

@@ -607,6 +607,21 @@ int aa_stream_sync(void* stream) {
     return AA_OK;
 }
 
+int aa_stream_query(void* stream, int32_t* done) {
+    if (!done) return fail(AA_ERR_INVALID, "aa_stream_query: NULL done");
+    const hipError_t e = hipStreamQuery(static_cast<hipStream_t>(stream));
+    if (e == hipSuccess) {
+        *done = 1;
+        return AA_OK;
+    }
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: clear it
+        *done = 0;
+        return AA_OK;
+    }
+    return fail(AA_ERR_DEVICE, "hipStreamQuery failed: %s", hipGetErrorString(e));
+}
+
 }  // extern "C"
 
 namespace {

@@ -194,6 +194,12 @@ int aa_trace_plain(aa_scene_t scene, const aa_trace_params *params, const float 
 /* Synchronise `stream` and report a kernel-side overflow flag, if any. */
 int aa_stream_sync(void *stream);
 
+/* Poll `stream` without blocking: *done = 1 once every call queued on it
+ * (traces, batch submissions) has finished, else 0.  With aa_stream_sync this
+ * is the poll / wait pair of a submitted batch (the reference's callers wait
+ * on futures of a thread pool, batch.py:170, api.py:511). */
+int aa_stream_query(void *stream, int32_t *done);
+
 /* ---- persistent batches (replaces the thread-pool fan-out of one trace per
  *      call: simulate_batch acoustic_agent/batch.py:139-184, run_many /
  *      run_batch / run_sources api.py:477-639, and the per-channel traces of

@@ -147,3 +147,25 @@ def test_planned_motion_equals_frame_by_frame_traces():
             assert rel_l1(got[t]["echogram"][k], one["echogram"][0]) <= 1e-5
             np.testing.assert_array_equal(got[t]["hit_counts"][k], one["hit_counts"][0])
             assert int(got[t]["active_count"][k]) == int(one["active_count"][0])
+
+
+def test_stream_query_polls_a_submission(requests):
+    # aa_stream_query / aa_stream_sync: the poll and wait of a submitted batch
+    import ctypes
+
+    import torch
+
+    from acoustic_agent_amd import _lib
+
+    lib = _lib.load()
+    engine = BatchEngine(0, max_traces=64)
+    s = torch.cuda.Stream(0)
+    res = engine.submit(requests[:6], stream=s)
+    done = ctypes.c_int32(-1)
+    _lib.check(lib.aa_stream_query(ctypes.c_void_p(s.cuda_stream), ctypes.byref(done)))
+    assert done.value in (0, 1)
+    _lib.check(lib.aa_stream_sync(ctypes.c_void_p(s.cuda_stream)))
+    _lib.check(lib.aa_stream_query(ctypes.c_void_p(s.cuda_stream), ctypes.byref(done)))
+    assert done.value == 1
+    assert len(res.wait()) == 6
+    engine.close()
