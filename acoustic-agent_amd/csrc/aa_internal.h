@@ -67,6 +67,7 @@ int set_error(int code, const char* fmt, ...);
 struct DeviceProps {
     int cu = 0;
     size_t lds_per_block = 0;
+    size_t lds_per_cu = 0;
     bool valid = false;
 };
 const DeviceProps& device_props(int device);
