@@ -748,17 +748,16 @@ extern "C" int aa_trace(aa_scene_t scene, const aa_trace_params* p, const float*
     lds_nodes = std::min({lds_nodes, num_nodes, (int)((lds_cap - stack_bytes) / sizeof(aa::WideNode))});
     if (!p->use_bvh) lds_nodes = 0;
     const size_t node_bytes = (size_t)lds_nodes * sizeof(aa::WideNode);
+    // LDS that keeps the workgroups the register budget allows resident (3
+    // waves per SIMD: 12 waves per CU); optional privatisations must fit in
+    // it (measured: a histogram that costs occupancy is 1.4-2.3x slower)
+    const size_t lds_resident = (props.lds_per_cu > 0 ? props.lds_per_cu : lds_cap) /
+                                (size_t)std::max(1, 12 * 64 / wg);
     bool lds_hist;
-    if (p->lds_histogram >= 0) {
+    if (p->lds_histogram >= 0)
         lds_hist = p->lds_histogram != 0;
-    } else {
-        // privatise the histogram only while the workgroups that the register
-        // budget allows (3 waves per SIMD: 12 waves per CU) still fit in LDS;
-        // measured: a histogram that costs occupancy is 1.4-2.3x slower
-        const int wg_per_cu = std::max(1, 12 * 64 / wg);
-        const size_t lds_cu = props.lds_per_cu > 0 ? props.lds_per_cu : lds_cap;
-        lds_hist = (stack_bytes + node_bytes + hist_bytes) * (size_t)wg_per_cu <= lds_cu;
-    }
+    else
+        lds_hist = stack_bytes + node_bytes + hist_bytes <= lds_resident;
     if (lds_hist && stack_bytes + node_bytes + hist_bytes > lds_cap) {
         if (p->lds_histogram > 0)
             return fail(AA_ERR_LIMIT, "LDS histogram of %zu bytes does not fit next to the stack", hist_bytes);
@@ -769,7 +768,8 @@ extern "C" int aa_trace(aa_scene_t scene, const aa_trace_params* p, const float*
     a.p = aa::make_ray_params(p, d_sources, d_directions, d_diffuse_bank, d_diffuse_random, d_diffuse_indices);
     a.out = aa::make_outputs(o, p);
     a.error_flag = scene->d_error;
-    a.lds_surf = (Ks == 1 && stack_bytes + node_bytes + (lds_hist ? hist_bytes : 0) + surf_bytes <= lds_cap) ? 1 : 0;
+    a.lds_surf = (Ks == 1 && stack_bytes + node_bytes + (lds_hist ? hist_bytes : 0) + surf_bytes <=
+                  std::max(lds_resident, stack_bytes + node_bytes + (lds_hist ? hist_bytes : 0))) ? 1 : 0;
     const size_t lds = stack_bytes + node_bytes + (lds_hist ? hist_bytes : 0) + (a.lds_surf ? surf_bytes : 0);
     if (lds > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes per workgroup", lds, lds_cap);
     const int grid_cap = (int)std::max<int64_t>(1, ((int64_t)p->num_rays + wg - 1) / wg);
