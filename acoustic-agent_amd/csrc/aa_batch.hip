@@ -294,13 +294,23 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
     }
     const int waves = wg / 64;
     const size_t lds_cap = props.lds_per_block > 0 ? props.lds_per_block : 65536;
-    const size_t surf_budget = 32 * 1024;
-    const int surf_cap = (max_surf > 0 && (size_t)waves * max_surf * 12 <= surf_budget) ? max_surf : 0;
+    // surface tallies only while the workgroups the register budget allows
+    // (12 waves per CU) stay resident next to the requested node slots
+    const size_t lds_resident = (props.lds_per_cu > 0 ? props.lds_per_cu : lds_cap) /
+                                (size_t)std::max(1, 12 * 64 / wg);
+    const size_t node_request = (size_t)waves * std::min(std::max(tuning->lds_nodes, 0), max_nodes) *
+                                sizeof(aa::WideNode);
+    const int surf_cap = (max_surf > 0 && aa::private_bytes(wg) + (size_t)waves * max_surf * 12 + node_request <=
+                                              lds_resident) ? max_surf : 0;
     const size_t base = aa::private_bytes(wg) + (size_t)waves * surf_cap * 12;
     if (base > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", base, lds_cap);
     // staged nodes only speed traversal up: take what the tuning asks for, as
     // far as it fits next to the stacks, queues and tallies
-    const int node_room = (int)((lds_cap - base) / ((size_t)waves * sizeof(aa::WideNode)));
+    // the kernel's static LDS (the per-wave cursors) comes on top of the
+    // dynamic request
+    const size_t static_lds = (size_t)waves * sizeof(unsigned);
+    if (base + static_lds > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", base, lds_cap);
+    const int node_room = (int)((lds_cap - base - static_lds) / ((size_t)waves * sizeof(aa::WideNode)));
     const int lds_nodes = std::min({std::max(tuning->lds_nodes, 0), max_nodes, node_room});
     const size_t lds = base + (size_t)waves * lds_nodes * sizeof(aa::WideNode);
 
