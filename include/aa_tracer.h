@@ -191,7 +191,9 @@ int aa_trace_plain(aa_scene_t scene, const aa_trace_params *params, const float 
                    const float *d_directions, const float *d_diffuse_bank, const float *d_diffuse_random,
                    const int32_t *d_diffuse_indices, const aa_trace_outputs *outputs, void *stream);
 
-/* Synchronise `stream` and report a kernel-side overflow flag, if any. */
+/* Synchronise `stream` (device errors come back as AA_ERR_DEVICE).  The
+ * traversal stack cannot overflow: aa_scene_create rejects trees deeper than
+ * the stack holds (AA_ERR_LIMIT). */
 int aa_stream_sync(void *stream);
 
 /* Poll `stream` without blocking: *done = 1 once every call queued on it
