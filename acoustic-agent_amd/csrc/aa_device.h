@@ -356,7 +356,6 @@ __device__ __forceinline__ void finish_ray(const RayParams& p, const Outputs& ou
     }
 }
 
-// Start `ray`; returns false if it has no bounce to trace (num_bounces == 0).
 // Visual candidate slot of `ray`, or -1 (steam_rt.py:4724-4729): rays with
 // ray % stride == 0 while ray / stride is below the limit.
 __device__ __forceinline__ int visual_slot(const Outputs& out, int ray) {
@@ -365,7 +364,9 @@ __device__ __forceinline__ int visual_slot(const Outputs& out, int ray) {
     return v < out.vis_limit ? v : -1;
 }
 
-// VISUAL: the caller may record visual candidates (the batch kernel never does).
+// Start lane task `task` (ray x source group); returns false if the ray has
+// no bounce to trace (num_bounces == 0).  VISUAL: the caller may record
+// visual candidates (the batch kernel never does).
 template <bool VISUAL = true>
 __device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out, int task, RayState& s,
                                           LaneStats& st) {
@@ -408,14 +409,17 @@ __device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out
 // bounce or query they are in.  When a query ends, the lane does the
 // reference's work for that point of the bounce loop (steam_rt.py:4743-4848)
 // and sets up its next query:
-//   bounce done -> termination tests, hit counts, then the first shadow
-//                  query of the hit (sources in ascending order), or the
-//                  continuation if no source needs one;
-//   shadow done -> deposit if unoccluded, then the next source's shadow
-//                  query, or the continuation;
+//   bounce done -> termination tests, hit counts, then one connection per
+//                  live source that needs a shadow query (sources in
+//                  ascending order), each parked in the wave's queue for an
+//                  idle lane (the ray's own lane runs it if the queue is
+//                  full), then the continuation;
+//   shadow done -> deposit if unoccluded; a lane that ran its own ray's
+//                  connection goes on with that ray, others become idle;
 //   continuation -> energy / path update, specular or diffuse direction, then
 //                  the next bounce query, or the end of the ray.
-// The arithmetic and its order are those of the one-ray-at-a-time loop.
+// The arithmetic and its order are those of the one-ray-at-a-time loop; only
+// the order of the float atomic sums changes.
 
 enum : int { kIdle = 0, kBounce = 1, kShadow = 2 };
 
