@@ -894,6 +894,86 @@ __device__ __forceinline__ bool close_bounce(const SceneView& sv, const RayParam
     return true;
 }
 
+// Multi-source hits: the candidate connections (ray, source) of the wave's
+// waiting rays are spread over its free lanes (lanes not traversing: their
+// query and connection registers are unused during a completion pass), each
+// free lane computing and parking one connection with the same arithmetic as
+// the ray's own lane would (connect() on the owner's ray and hit, read with
+// lane shuffles).  Without it a ray makes its group's connections one per
+// loop round (about 6 rounds of ~10 lanes per pass).  Takes at most as many
+// as the queue has room for; the rest stay in the owners' c.nee for the
+// per-lane loop.  Every lane of the wave calls it.
+__device__ __forceinline__ void spread_connections(const SceneView& sv, const RayParams& p, const RayState& s,
+                                                   HitContext& c, bool owner, bool free_lane, Query& q,
+                                                   Connection& cn, float* qu, int q_head, int& q_count,
+                                                   int lane, unsigned long long below, LaneStats& st) {
+    const int cnt = owner ? __popcll(c.nee) : 0;
+    int prefix = 0, total = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) {  // cnt <= 64: 7-bit wave prefix sum from ballots
+        const unsigned long long bb = __ballot((cnt >> b) & 1);
+        prefix += __popcll(bb & below) << b;
+        total += __popcll(bb) << b;
+    }
+    const unsigned long long fb = __ballot(free_lane);
+    const int take = min(min(total, __popcll(fb)), kQueueCap - q_count);
+    if (take <= 0) return;
+    const int r = __popcll(fb & below);  // this lane's rank among the free lanes
+    const bool helper = free_lane && r < take;
+    // owner of candidate r: the first lane whose inclusive prefix exceeds r
+    const int incl = prefix + cnt;
+    int lo = 0, hi = 63;
+#pragma unroll
+    for (int step = 0; step < 6; ++step) {
+        const int mid = (lo + hi) >> 1;
+        const int v = __shfl(incl, mid);
+        if (v > r)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    const int o = helper ? lo : lane;
+    const int o_prefix = __shfl(prefix, o);
+    const unsigned o_lo = __shfl((unsigned)c.nee, o), o_hi = __shfl((unsigned)(c.nee >> 32), o);
+    RayState ts;
+    HitContext tc;
+    ts.dx = __shfl(s.dx, o);
+    ts.dy = __shfl(s.dy, o);
+    ts.dz = __shfl(s.dz, o);
+    ts.d0x = __shfl(s.d0x, o);
+    ts.d0y = __shfl(s.d0y, o);
+    ts.d0z = __shfl(s.d0z, o);
+#pragma unroll
+    for (int b = 0; b < kBands; ++b) ts.A[b] = __shfl(s.A[b], o);
+    ts.accum = __shfl(s.accum, o);
+    tc.hx = __shfl(c.hx, o);
+    tc.hy = __shfl(c.hy, o);
+    tc.hz = __shfl(c.hz, o);
+    tc.nx = __shfl(c.nx, o);
+    tc.ny = __shfl(c.ny, o);
+    tc.nz = __shfl(c.nz, o);
+    tc.t = __shfl(c.t, o);
+    tc.surf = __shfl(c.surf, o);
+    bool parked = false;
+    if (helper) {
+        unsigned long long m = ((unsigned long long)o_hi << 32) | o_lo;
+        for (int j = r - o_prefix; j > 0; --j) m &= m - 1ull;  // the j-th pending source, ascending
+        const int k = __builtin_ctzll(m);
+        if (connect(sv, p, ts, tc, k, q, cn)) {
+            ++st.q_s;
+            parked = cn.bin >= 0;
+        }
+    }
+    const unsigned long long pb = __ballot(parked);
+    if (parked) park(qu, (q_head + q_count + __popcll(pb & below)) & (kQueueCap - 1), q, cn);
+    q_count += __popcll(pb);
+    // owners drop the candidates handed out (the lowest ones)
+    if (owner) {
+        int taken = min(max(take - prefix, 0), cnt);
+        while (taken-- > 0) c.nee &= c.nee - 1ull;
+    }
+}
+
 // Hand out tasks to the lanes of this wave that need one: one atomic per
 // wave on `cursor`; lanes past `limit` get -1.
 __device__ __forceinline__ int claim_rays(bool need, unsigned* cursor, int limit) {
@@ -965,6 +1045,12 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
                     nee = own;
                     if (!own) phase = kIdle;
                 }
+            }
+            if (p.num_sources > 1) {
+                // two spreading rounds cover a pass's connections in practice
+                for (int round = 0; round < 2; ++round)
+                    spread_connections(sv, p, s, c, nee && c.nee != 0ull, !walking, q, cn, qu, q_head, q_count,
+                                       lane, below, st);
             }
             // park the new connections while the queue has room; a lane whose
             // connection does not fit resolves it itself before going on
