@@ -24,6 +24,9 @@
  *                      NEE + deposit 4770-4826, reduction 4851-4871); the
  *                      energy expression follows the operand order of the FP32
  *                      kernel steam_rt_cuda.py:663-677 in both precisions.
+ *                      Visual candidates (hit points, surfaces, strongest
+ *                      contribution) follow steam_rt.py:4718-4731, 4765-4769
+ *                      and 4821-4824.
  */
 #include <math.h>
 #include <stdint.h>

@@ -20,6 +20,11 @@ Pins (see tests/test_oracle.py):
                 FP32 vs FP64 within the reference's tolerance
                 (tests/test_accelerators.py:31-53), FOA direction invariant
                 (tests/test_steam_audio_parity.py:329-357)
+  visual        candidate surfaces = the hit log while the ray lives, step
+                lengths = logged t (+- the hit offset); a ray traced alone:
+                strongest contribution = largest band-summed bin, and its
+                path length maps back to that bin
+Every function is pinned by at least one of these; none is unpinned.
 """
 from __future__ import annotations
 
