@@ -245,14 +245,14 @@ __global__ __launch_bounds__(WG) void plain_kernel(const PlainArgs a) {
     Lds lds = {};  // every tally goes to global atomics
     if (task < a.p.num_rays * a.p.groups) {
         RayState s;
-        if (start_ray(a.p, a.out, task, s, st)) {
+        if (start_ray<false>(a.p, a.out, task, s, st)) {  // no visual candidates here
             Query q;
             begin_query(q, s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, kBig);
             ++st.q_b;
             while (true) {
                 q.h = plain_closest(a, q.r, st.nb, st.tb);
                 HitContext c;
-                if (!close_bounce(a.sv, a.p, a.out, lds, s, c, q, st)) break;
+                if (!close_bounce<false>(a.sv, a.p, a.out, lds, s, c, q, st)) break;
                 unsigned long long m = c.nee;
                 while (m) {  // next-event connections (steam_rt.py:4770-4826)
                     const int k = __builtin_ctzll(m);
