@@ -399,14 +399,14 @@ __device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out
 }
 
 // ---------------------------------------------------------------------------
-// Per-lane state machine: one loop iteration = one traversal step
+// Per-lane state machine: one loop round = a few traversal steps
 // ---------------------------------------------------------------------------
 //
 // A lane is IDLE (needs a ray), in a BOUNCE query (closest hit of the ray's
 // current segment) or in a SHADOW query (any hit towards one source).  Every
-// iteration advances each busy lane's query by one BVH node (with its leaf
-// triangles), so the lanes of a wave run the same traversal code whatever
-// bounce or query they are in.  When a query ends, the lane does the
+// round advances each busy lane's query by up to kStepsPerRound BVH nodes
+// (each with its leaf triangles), so the lanes of a wave run the same
+// traversal code whatever bounce or query they are in.  When a query ends, the lane does the
 // reference's work for that point of the bounce loop (steam_rt.py:4743-4848)
 // and sets up its next query:
 //   bounce done -> termination tests, hit counts, then one connection per
