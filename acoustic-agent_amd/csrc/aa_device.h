@@ -2,7 +2,7 @@
 // persistent batch kernel (aa_batch.hip): BVH traversal, triangle test,
 // counter RNG and the bounce loop of the reference tracer
 // (acoustic_agent/steam_rt.py:4724-4849), run as a per-lane state machine
-// (one traversal step per iteration) with a per-wave queue of shadow
+// (a few traversal steps per loop round) with a per-wave queue of shadow
 // connections.
 //
 // Floating-point contract: built with -ffp-contract=off and IEEE division /
