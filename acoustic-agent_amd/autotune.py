@@ -1,10 +1,10 @@
 """Autotuner of launch shapes, cached per device architecture and size class.
 
 Tunables (none changes results, only speed):
-  batch:  workgroup size, rays per work item (packet), top BVH nodes staged
-          in LDS per wave, persistent workgroups per CU;
-  single: workgroup size, LDS-privatised histogram, staged top nodes,
-          persistent workgroups per CU.
+  batch:  workgroup size, rays per work item (packet), persistent
+          workgroups per CU;
+  single: workgroup size, LDS-privatised histogram, persistent workgroups
+          per CU.
 Each candidate is timed with HIP events on a sample of the real work; the
 winner is stored under (arch, size class) in memory and, if ``cache_path`` is
 given, in a JSON file.  A size class is (log2 of triangles, log2 of rays).
@@ -25,13 +25,11 @@ from .batch import BatchEngine, BatchTuning, TraceRequest
 BATCH_SPACE = {
     "block_size": (128, 256),
     "packet_rays": (0, 1024, 4096),
-    "lds_nodes": (0, 21, 85),
     "workgroups_per_cu": (0,),
 }
 TRACE_SPACE = {
     "block_size": (64, 128, 256),
     "lds_histogram": (0, 1),
-    "lds_nodes": (0, 21, 85),
     "workgroups_per_cu": (0,),
 }
 

@@ -86,7 +86,7 @@ class TraceRequest:
 class BatchTuning:
     block_size: int = 256
     packet_rays: int = 0  # 0: about two work items per resident wave
-    lds_nodes: int = 21
+    lds_nodes: int = 0  # ignored by the library (nodes are read through L1/L2); kept for the C struct
     workgroups_per_cu: int = 0
     use_graph: bool = False
 

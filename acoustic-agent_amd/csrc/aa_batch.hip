@@ -86,31 +86,26 @@ __device__ __forceinline__ T uniform_copy(const T* src) {
 }
 
 // Work items are claimed per wave, so that a wave whose rays run long does
-// not hold up the other waves of its workgroup; each wave has its own slot of
-// staged BVH nodes, its own surface tallies and its own ray cursor in LDS.
+// not hold up the other waves of its workgroup; each wave has its own surface
+// tallies and its own ray cursor in LDS.
 // Registers are held to 3 waves per SIMD, as for the single-trace kernel.
 template <int WG>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void batch_kernel(BatchHeader* hdr, const BatchTrace* traces, int lds_nodes_cap,
-                                                   int surf_cap) {
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void batch_kernel(BatchHeader* hdr, const BatchTrace* traces, int surf_cap) {
     constexpr int kWaves = WG / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     unsigned short* s_stack = reinterpret_cast<unsigned short*>(smem);
     float* s_queue = reinterpret_cast<float*>(smem + stack_only_bytes(WG));
-    float4* s_nodes = reinterpret_cast<float4*>(smem + private_bytes(WG));
-    int* s_surf = reinterpret_cast<int*>(s_nodes + (size_t)kWaves * lds_nodes_cap * 8);
+    int* s_surf = reinterpret_cast<int*>(smem + private_bytes(WG));
     __shared__ unsigned s_cursor[kWaves];
 
     const int wave = (int)__builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar LDS addresses
     const int lane = (int)(threadIdx.x & 63);
-    float4* my_nodes = s_nodes + (size_t)wave * lds_nodes_cap * 8;
     int* my_hits = s_surf + (size_t)wave * surf_cap * 3;
     int* my_contrib = my_hits + surf_cap;
     float* my_energy = reinterpret_cast<float*>(my_contrib + surf_cap);
     const int num_traces = hdr->num_traces;
     const int total = hdr->total_packets;
     const int packet_rays = hdr->packet_rays;
-    const float4* staged = nullptr;  // scene whose top nodes sit in my_nodes
-    int staged_count = 0;
     while (true) {
         unsigned claimed = 0u;
         if (lane == 0) claimed = atomicAdd(&hdr->queue_head, 1u);
@@ -128,17 +123,11 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
                 hi = mid - 1;
         }
         const BatchTrace& bt = traces[lo];
-        SceneView sv = uniform_copy(&bt.sv);
+        const SceneView sv = uniform_copy(&bt.sv);
         const RayParams p = uniform_copy(&bt.p);
         const Outputs out = uniform_copy(&bt.out);
         int* const error_flag = uniform_copy(&bt.error_flag);
         const int first_packet = __builtin_amdgcn_readfirstlane(bt.first_packet);
-        sv.lds_nodes = min(lds_nodes_cap, sv.use_bvh ? sv.num_nodes : 0);
-        if (sv.nodes != staged || sv.lds_nodes != staged_count) {
-            for (int i = lane; i < sv.lds_nodes * 8; i += 64) my_nodes[i] = sv.nodes[i];
-            staged = sv.nodes;
-            staged_count = sv.lds_nodes;
-        }
         const bool lds_surf = __builtin_amdgcn_readfirstlane(bt.lds_surf) != 0;
         if (lds_surf) {
             for (int i = lane; i < sv.num_surfaces; i += 64) {
@@ -155,7 +144,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
         Lds lds;
         lds.stack = s_stack;
         lds.queue = s_queue;
-        lds.nodes = my_nodes;
         lds.hist = nullptr;
         lds.hits = lds_surf ? my_hits : nullptr;
         lds.contrib = lds_surf ? my_contrib : nullptr;
@@ -200,14 +188,14 @@ size_t table_size(int max_traces) {
 }
 
 template <int WG>
-hipError_t launch_batch(aa::BatchHeader* hdr, const aa::BatchTrace* traces, int grid, size_t lds, int lds_nodes,
-                        int surf_cap, hipStream_t stream) {
+hipError_t launch_batch(aa::BatchHeader* hdr, const aa::BatchTrace* traces, int grid, size_t lds, int surf_cap,
+                        hipStream_t stream) {
     auto kern = aa::batch_kernel<WG>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), lds, stream, hdr, traces, lds_nodes, surf_cap);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), lds, stream, hdr, traces, surf_cap);
     return hipGetLastError();
 }
 
@@ -284,35 +272,26 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
     const aa::DeviceProps& props = aa::device_props(b->device);
     if (!props.valid) return fail(AA_ERR_DEVICE, "cannot query device %d", b->device);
 
-    // LDS plan shared by every trace of the launch: per-wave node slots and
-    // per-wave surface tallies (histograms go to global atomics)
-    int max_nodes = 0, max_surf = 0;
+    // LDS plan shared by every trace of the launch: per-wave surface tallies
+    // (histograms go to global atomics)
+    int max_surf = 0;
     for (int32_t i = 0; i < count; ++i) {
         const aa_batch_trace& t = traces[i];
-        if (t.params.use_bvh) max_nodes = std::max(max_nodes, (int)t.scene->wide.nodes.size());
         if (t.params.num_sources == 1) max_surf = std::max(max_surf, t.scene->num_surfaces);
     }
     const int waves = wg / 64;
     const size_t lds_cap = props.lds_per_block > 0 ? props.lds_per_block : 65536;
     // surface tallies only while the workgroups the register budget allows
-    // (12 waves per CU) stay resident next to the requested node slots
+    // (12 waves per CU) stay resident
     const size_t lds_resident = (props.lds_per_cu > 0 ? props.lds_per_cu : lds_cap) /
                                 (size_t)std::max(1, 12 * 64 / wg);
-    const size_t node_request = (size_t)waves * std::min(std::max(tuning->lds_nodes, 0), max_nodes) *
-                                sizeof(aa::WideNode);
-    const int surf_cap = (max_surf > 0 && aa::private_bytes(wg) + (size_t)waves * max_surf * 12 + node_request <=
-                                              lds_resident) ? max_surf : 0;
-    const size_t base = aa::private_bytes(wg) + (size_t)waves * surf_cap * 12;
-    if (base > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", base, lds_cap);
-    // staged nodes only speed traversal up: take what the tuning asks for, as
-    // far as it fits next to the stacks, queues and tallies
+    const int surf_cap =
+        (max_surf > 0 && aa::private_bytes(wg) + (size_t)waves * max_surf * 12 <= lds_resident) ? max_surf : 0;
+    const size_t lds = aa::private_bytes(wg) + (size_t)waves * surf_cap * 12;
     // the kernel's static LDS (the per-wave cursors) comes on top of the
     // dynamic request
     const size_t static_lds = (size_t)waves * sizeof(unsigned);
-    if (base + static_lds > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", base, lds_cap);
-    const int node_room = (int)((lds_cap - base - static_lds) / ((size_t)waves * sizeof(aa::WideNode)));
-    const int lds_nodes = std::min({std::max(tuning->lds_nodes, 0), max_nodes, node_room});
-    const size_t lds = base + (size_t)waves * lds_nodes * sizeof(aa::WideNode);
+    if (lds + static_lds > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", lds, lds_cap);
 
     int per_cu = tuning->workgroups_per_cu;
     if (per_cu <= 0) {
@@ -349,7 +328,7 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
         const aa_batch_trace& t = traces[i];
         aa::BatchTrace& d = h_tr[i];
         std::memset(&d, 0, sizeof(d));
-        d.sv = aa::make_scene_view(t.scene, t.params.use_bvh != 0, 0);
+        d.sv = aa::make_scene_view(t.scene, t.params.use_bvh != 0);
         d.p = aa::make_ray_params(&t.params, t.d_sources, t.d_directions, t.d_diffuse_bank, t.d_diffuse_random,
                                   t.d_diffuse_indices);
         d.out = aa::make_outputs(&t.outputs, &t.params);
@@ -390,7 +369,7 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
         const int g = tuning->use_graph ? persistent_grid : grid;
 #define AA_LAUNCH(W)                                                             \
     case W:                                                                      \
-        e = launch_batch<W>(d_hdr, d_tr, g, lds, lds_nodes, surf_cap, s);        \
+        e = launch_batch<W>(d_hdr, d_tr, g, lds, surf_cap, s);                   \
         break;
         switch (wg) {
             AA_LAUNCH(64)
@@ -403,7 +382,7 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
     };
 
     if (tuning->use_graph) {
-        const auto key = std::make_tuple(wg, lds_nodes, 0, surf_cap, persistent_grid, 0, stream);
+        const auto key = std::make_tuple(wg, 0, 0, surf_cap, persistent_grid, 0, stream);
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
             hipStream_t cap;

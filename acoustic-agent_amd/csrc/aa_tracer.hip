@@ -3,8 +3,7 @@
 //
 // One lane traces one listener ray through up to B bounces
 // (acoustic_agent/steam_rt.py:4724-4849, device code in aa_device.h):
-// closest hit through a 4-wide BVH (top levels staged in LDS, short per-lane
-// stack in LDS), listener/source sphere termination, a next-event shadow
+// closest hit through a 4-wide BVH (short per-lane stack in LDS), listener/source sphere termination, a next-event shadow
 // query towards each source (any-hit with early exit, equal to the
 // reference's closest-hit test `EPS < t < D - EPS`, steam_rt.py:4788-4792),
 // the BRDF-weighted six-band deposit into the time histogram
@@ -86,15 +85,13 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void tr
     const int S = a.sv.num_surfaces;
     unsigned short* s_stack = reinterpret_cast<unsigned short*>(smem);
     float* s_queue = reinterpret_cast<float*>(smem + stack_only_bytes(WG));
-    float4* s_nodes = reinterpret_cast<float4*>(smem + private_bytes(WG));
-    float* s_hist = reinterpret_cast<float*>(s_nodes + (size_t)a.sv.lds_nodes * 8);
+    float* s_hist = reinterpret_cast<float*>(smem + private_bytes(WG));
     const int hist_words = LDS_HIST ? a.p.num_sources * kBands * a.p.num_bins : 0;
     int* s_hits = reinterpret_cast<int*>(s_hist + hist_words);
     int* s_contrib = s_hits + (a.lds_surf ? S : 0);
     float* s_senergy = reinterpret_cast<float*>(s_contrib + (a.lds_surf ? S : 0));
 
     const int tid = threadIdx.x;
-    for (int i = tid; i < a.sv.lds_nodes * 8; i += WG) s_nodes[i] = a.sv.nodes[i];
     if (LDS_HIST)
         for (int i = tid; i < hist_words; i += WG) s_hist[i] = 0.0f;
     if (a.lds_surf) {
@@ -109,7 +106,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void tr
     Lds lds;
     lds.stack = s_stack;
     lds.queue = s_queue;
-    lds.nodes = s_nodes;
     lds.hist = LDS_HIST ? s_hist : nullptr;
     lds.hits = a.lds_surf ? s_hits : nullptr;
     lds.contrib = a.lds_surf ? s_contrib : nullptr;
@@ -732,22 +728,11 @@ extern "C" int aa_trace(aa_scene_t scene, const aa_trace_params* p, const float*
     // launch shape
     int wg = p->block_size > 0 ? p->block_size : 256;
     if (wg != 64 && wg != 128 && wg != 256 && wg != 512) return fail(AA_ERR_INVALID, "block_size must be 64..512");
-    const int num_nodes = (int)scene->wide.nodes.size();
-    int lds_nodes = p->lds_nodes >= 0 ? p->lds_nodes : 0;
-    if (p->lds_nodes < 0) {
-        // default: the levels that fit in 4 KB
-        for (size_t l = 0; l < scene->wide.level_end.size(); ++l)
-            if ((size_t)scene->wide.level_end[l] * sizeof(aa::WideNode) <= 4096) lds_nodes = scene->wide.level_end[l];
-    }
     const size_t stack_bytes = aa::private_bytes(wg);  // stacks + connection queues
     const size_t hist_bytes = (size_t)Ks * AA_NUM_BANDS * K * sizeof(float);
     const size_t surf_bytes = (size_t)S * 3 * sizeof(int);
     const size_t lds_cap = props.lds_per_block > 0 ? props.lds_per_block : 65536;
     if (stack_bytes > lds_cap) return fail(AA_ERR_LIMIT, "LDS stacks of %zu bytes exceed %zu", stack_bytes, lds_cap);
-    // staged nodes only speed traversal up: as many as asked for that fit
-    lds_nodes = std::min({lds_nodes, num_nodes, (int)((lds_cap - stack_bytes) / sizeof(aa::WideNode))});
-    if (!p->use_bvh) lds_nodes = 0;
-    const size_t node_bytes = (size_t)lds_nodes * sizeof(aa::WideNode);
     // LDS that keeps the workgroups the register budget allows resident (3
     // waves per SIMD: 12 waves per CU); optional privatisations must fit in
     // it (measured: a histogram that costs occupancy is 1.4-2.3x slower)
@@ -757,20 +742,20 @@ extern "C" int aa_trace(aa_scene_t scene, const aa_trace_params* p, const float*
     if (p->lds_histogram >= 0)
         lds_hist = p->lds_histogram != 0;
     else
-        lds_hist = stack_bytes + node_bytes + hist_bytes <= lds_resident;
-    if (lds_hist && stack_bytes + node_bytes + hist_bytes > lds_cap) {
+        lds_hist = stack_bytes + hist_bytes <= lds_resident;
+    if (lds_hist && stack_bytes + hist_bytes > lds_cap) {
         if (p->lds_histogram > 0)
             return fail(AA_ERR_LIMIT, "LDS histogram of %zu bytes does not fit next to the stack", hist_bytes);
         lds_hist = false;
     }
     aa::TraceArgs a;
-    a.sv = aa::make_scene_view(scene, p->use_bvh != 0, lds_nodes);
+    a.sv = aa::make_scene_view(scene, p->use_bvh != 0);
     a.p = aa::make_ray_params(p, d_sources, d_directions, d_diffuse_bank, d_diffuse_random, d_diffuse_indices);
     a.out = aa::make_outputs(o, p);
     a.error_flag = scene->d_error;
-    a.lds_surf = (Ks == 1 && stack_bytes + node_bytes + (lds_hist ? hist_bytes : 0) + surf_bytes <=
-                  std::max(lds_resident, stack_bytes + node_bytes + (lds_hist ? hist_bytes : 0))) ? 1 : 0;
-    const size_t lds = stack_bytes + node_bytes + (lds_hist ? hist_bytes : 0) + (a.lds_surf ? surf_bytes : 0);
+    a.lds_surf = (Ks == 1 && stack_bytes + (lds_hist ? hist_bytes : 0) + surf_bytes <=
+                  std::max(lds_resident, stack_bytes + (lds_hist ? hist_bytes : 0))) ? 1 : 0;
+    const size_t lds = stack_bytes + (lds_hist ? hist_bytes : 0) + (a.lds_surf ? surf_bytes : 0);
     if (lds > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes per workgroup", lds, lds_cap);
     const int grid_cap = (int)std::max<int64_t>(1, ((int64_t)p->num_rays + wg - 1) / wg);
 
@@ -806,7 +791,7 @@ extern "C" int aa_trace_plain(aa_scene_t scene, const aa_trace_params* p, const 
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
     AA_HIP(hipSetDevice(scene->device));
     aa::PlainArgs a;
-    a.sv = aa::make_scene_view(scene, p->use_bvh != 0, 0);
+    a.sv = aa::make_scene_view(scene, p->use_bvh != 0);
     a.nodes = reinterpret_cast<const float4*>(scene->d_plain_nodes);
     a.tris = reinterpret_cast<const float4*>(scene->d_plain_tris);
     a.num_nodes = scene->plain_node_count;

@@ -139,7 +139,7 @@ typedef struct {
     float dipole_power;
     int32_t block_size;          /* 0: tuned default; else 64/128/256/512 */
     int32_t lds_histogram;       /* -1: tuned default; 0: global atomics; 1: LDS privatised */
-    int32_t lds_nodes;           /* -1: tuned default; else top 4-wide nodes staged in LDS */
+    int32_t lds_nodes;           /* ignored: nodes are read through L1/L2 (field kept for layout) */
     int32_t workgroups_per_cu;   /* persistent grid = CUs x this (0: occupancy limit) */
     int32_t sources_per_lane;    /* multi-source traces: sources sharing one lane's trajectory
                                     (0: 8); the trace runs num_rays x ceil(Ks / this) lane tasks */
@@ -229,7 +229,7 @@ typedef struct {
 typedef struct {
     int32_t block_size;         /* 64, 128, 256 or 512 lanes per workgroup */
     int32_t packet_rays;        /* rays per work item */
-    int32_t lds_nodes;          /* top 4-wide nodes staged in LDS per scene */
+    int32_t lds_nodes;          /* ignored: nodes are read through L1/L2 (field kept for layout) */
     int32_t lds_histogram;      /* ignored: batch histograms use global atomics (kept for layout) */
     int32_t workgroups_per_cu;  /* persistent grid = CUs x this (0: from LDS use) */
     int32_t use_graph;          /* 1: capture copy + zero + kernel in a hipGraph and replay it */
