@@ -505,11 +505,6 @@ template <int WG>
 __device__ __forceinline__ void traversal_step(const SceneView& sv, unsigned short* stk,
                                                Query& q, bool shadow, unsigned& node_cnt, unsigned& tri_cnt,
                                                int* error_flag) {
-    if (!sv.use_bvh) {
-        test_leaf(sv, 0, sv.num_tris, q, shadow, tri_cnt);
-        q.node = -1;
-        return;
-    }
     ++node_cnt;
     float4 v[7];
     load_node(sv, q.node, v);
@@ -1120,9 +1115,15 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
             // completions again: the round's bookkeeping (ballots, the
             // completion test, reloads of spilled scalars) is paid once per
             // several visits.  A lane whose query ends drops out and waits.
+            if (!sv.use_bvh) {
+                // brute force (the reference's use_bvh=False): every triangle
+                test_leaf(sv, 0, sv.num_tris, q, shadow, tris);
+                q.node = -1;
+            } else {
 #pragma unroll 1
-            for (int k = 0; k < kStepsPerRound && q.node >= 0; ++k)
-                traversal_step<WG>(sv, stk, q, shadow, nodes, tris, error_flag);
+                for (int k = 0; k < kStepsPerRound && q.node >= 0; ++k)
+                    traversal_step<WG>(sv, stk, q, shadow, nodes, tris, error_flag);
+            }
             if (shadow) {
                 st.ns += nodes;
                 st.ts += tris;
