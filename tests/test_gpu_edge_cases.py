@@ -159,3 +159,29 @@ def test_visual_candidates_in_the_field_and_rejected_elsewhere(shoebox):
     case = Case(shoebox, num_rays=512, num_bounces=10, num_bins=500, seed=1)
     with pytest.raises(ValueError, match="aa_trace only"):
         run_gpu(case, visual_limit=8, visual_stride=4, plain=True)
+
+
+def test_fdn_toggle_changes_the_rendered_tail_of_a_gpu_traced_room():
+    # tests/test_steam_audio_parity.py:561-583 through the host mirror: the
+    # 5 x 4 x 2.8 m room traced on the GPU (1024 rays x 24 bounces), rendered
+    # with and without the FDN late tail; the change is measured against the
+    # tail's own energy (the traced reflections alone have another scale than
+    # the reference's full-solver RIR)
+    from dataclasses import replace
+
+    from acoustic_agent_amd.models import SimConfig
+    from acoustic_agent_amd.rir import render_band_rirs, render_rir
+    from acoustic_agent_amd.steam_rt import trace_energy_field
+
+    room = scenes.shoebox((5.0, 4.0, 2.8), 0)
+    cfg = SimConfig(fs=8000, duration_s=0.9, rt_duration_s=0.9, rt_num_rays=1024, rt_num_bounces=24,
+                    rt_visual_num_rays=32, rt_visual_num_bounces=2, hybrid_transition_s=0.3)
+    field = trace_energy_field(room, np.asarray((1.0, 1.0, 1.4)), np.asarray((4.0, 3.0, 1.4)), cfg)
+    enabled = render_rir(field, cfg).astype(np.float64)
+    disabled = render_rir(field, replace(cfg, late_tail=False)).astype(np.float64)
+    _, meta = render_band_rirs(field, cfg)
+    start = int(float(meta["transition_start_s"]) * cfg.fs)
+    assert meta["model"] == "steam_style_16_line_hadamard_fdn" and meta["applied"] is True
+    tail_energy = float(np.sum(np.square(disabled[start:])))
+    assert tail_energy > 0.0
+    assert float(np.sum(np.square(enabled[start:] - disabled[start:]))) > 0.01 * tail_energy
