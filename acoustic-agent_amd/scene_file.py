@@ -76,9 +76,11 @@ def save_scene(path: str | os.PathLike, mesh: TriangleMesh, reflection: np.ndarr
     c = np.ascontiguousarray(scattering, dtype=np.float32).reshape(-1)
     if r.shape[0] != mesh.num_surfaces or c.shape[0] != mesh.num_surfaces:
         raise ValueError("reflection must be [S,6] and scattering [S] for the mesh's S surfaces")
-    names = "\n".join(surface_names).encode("utf-8")
     if surface_names and len(surface_names) != mesh.num_surfaces:
         raise ValueError("surface_names must name every surface")
+    if any("\n" in name for name in surface_names):
+        raise ValueError("surface names must not contain a newline")
+    names = "\n".join(surface_names).encode("utf-8")
     h = content_hash(v, s, n, r, c, leaf_size)
     header = _HEADER.pack(MAGIC, VERSION, v.shape[0], r.shape[0], int(leaf_size), h, len(names), 0)
     tmp = f"{os.fspath(path)}.tmp"
@@ -93,7 +95,8 @@ def save_scene(path: str | os.PathLike, mesh: TriangleMesh, reflection: np.ndarr
 
 def load_scene(path: str | os.PathLike) -> SceneFile:
     """Read and verify ``path``."""
-    data = open(path, "rb").read()
+    with open(path, "rb") as f:
+        data = f.read()
     if len(data) < _HEADER.size:
         raise ValueError(f"{path}: shorter than a scene header")
     magic, version, T, S, leaf_size, h, name_bytes, _ = _HEADER.unpack_from(data, 0)

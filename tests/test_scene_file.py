@@ -69,3 +69,5 @@ def test_scene_file_rejects_damage(tmp_path):
             load_scene(path)
     with pytest.raises(ValueError):
         save_scene(path, mesh, r[:1], c[:1])
+    with pytest.raises(ValueError):  # names are newline-separated in the file
+        save_scene(path, mesh, r, c, surface_names=("wall", "floor\ntile"))
