@@ -889,7 +889,7 @@ __device__ __forceinline__ bool close_bounce(const SceneView& sv, const RayParam
 // the ray's own lane would (connect() on the owner's ray and hit, read with
 // lane shuffles).  Without it a ray makes its group's connections one per
 // loop round (about 6 rounds of ~10 lanes per pass on 64-keyframe traces;
-// measured C3 +21 %, single-source 2M rays +2 %).  Takes at most as many
+// measured: batched 64-keyframe traces +21 %, single-source 2M rays +2 %).  Takes at most as many
 // as the queue has room for; the rest stay in the owners' c.nee for the
 // per-lane loop.  Every lane of the wave calls it.
 __device__ __forceinline__ void spread_connections(const SceneView& sv, const RayParams& p, const RayState& s,
@@ -1047,7 +1047,8 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
                 const bool spread = nee && c.nee != 0ull &&
                                     !(VISUAL && out.vis_limit > 0 && visual_slot(out, s.ray) >= 0);
                 // two spreading rounds cover a pass's connections in practice
-                // (measured: one round C3 -6 %, three no better than two)
+                // (measured: one round -6 % on batched 64-keyframe traces,
+                // three no better than two)
                 for (int round = 0; round < 2; ++round)
                     spread_connections(sv, p, s, c, spread && c.nee != 0ull, !walking, q, cn, qu, q_head, q_count,
                                        lane, below, st);

@@ -102,8 +102,8 @@ def _batch_vs_plain(requests, label: str, steps: int, warmup: int, plain_sample:
     rays = sum(r.num_rays for r in requests)
     queries = sum(x["counters"]["bounce_queries"] + x["counters"]["shadow_queries"] for x in res)
     med = float(np.median(times))
-    # one-scene-per-launch baselines on a sample of single-source traces (a
-    # keyframe trace of C3 is one source): the plain kernel of the reference
+    # one-scene-per-launch baselines on a sample of single-source traces (for
+    # the keyframe workload, one keyframe each): the plain kernel of the reference
     # GPU backend's execution model, and this build's persistent single-trace
     # kernel
     units = [dataclasses.replace(r, sources=np.asarray(r.sources).reshape(-1, 3)[k:k + 1], tag=(r.tag, k))
@@ -151,7 +151,7 @@ def run_batched(args) -> int:
                 f"scene), 100k rays, {args.bounces} bounces, 1.0 s @ 1 ms, hipGraph replay"
     build_s = time.perf_counter() - t
     _progress(f"{len(reqs)} requests built in {build_s:.1f} s")
-    # the tuning sample is a few traces' worth of lane tasks (a C3 trace carries 64 sources)
+    # the tuning sample is a few traces' worth of lane tasks (a keyframe trace carries 64 sources)
     out = _batch_vs_plain(reqs, label, args.steps, args.warmup, args.plain_sample, args.workload == "c3",
                           not args.no_tune, tune_sample=64 if args.workload == "c2" else 4)
     out["scene_build_s"] = round(build_s, 2)
