@@ -68,6 +68,7 @@
         int32_t *log_prim;      /* [log_rays, log_bounces] or NULL */                   \
         int32_t *log_surface;                                                           \
         double *log_t;                                                                  \
+        double *log_point;      /* [log_rays, log_bounces, 3] hit point o + t d, or NULL */ \
         REALT *visual_points;      /* [V,B,3] offset hit points, or NULL */             \
         int32_t *visual_surfaces;  /* [V,B] */                                          \
         int32_t *visual_rays;      /* [V] */                                            \
@@ -84,7 +85,9 @@
                        ao_out_##SUF *out);                                              \
     int ao_closest_hit_##SUF(const ao_scene_##SUF *scene, const REALT origin[3],        \
                              const REALT direction[3], int32_t *prim, int32_t *surface, \
-                             REALT *t, REALT normal[3]);
+                             REALT *t, REALT normal[3]);                                \
+    void ao_diffuse_direction_##SUF(const REALT sample[3], const REALT normal[3],       \
+                                    REALT out[3]);
 
 AO_DECLARE(double, f64)
 AO_DECLARE(float, f32)

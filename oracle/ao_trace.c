@@ -385,6 +385,11 @@ static void diffuse_direction(const REAL s[3], REAL nx, REAL ny, REAL nz, REAL o
     out[0] = dx; out[1] = dy; out[2] = dz;
 }
 
+void CAT(ao_diffuse_direction, SUF)(const REAL sample[3], const REAL normal[3], REAL out[3])
+{
+    diffuse_direction(sample, normal[0], normal[1], normal[2], out);
+}
+
 int CAT(ao_closest_hit, SUF)(const SCENE *sc, const REAL origin[3], const REAL direction[3], int32_t *prim,
                              int32_t *surface, REAL *t, REAL normal[3])
 {
@@ -490,6 +495,12 @@ int CAT(ao_trace, SUF)(const SCENE *sc, const PARAMS *p, OUTS *out)
                 out->log_prim[li] = prim;
                 out->log_surface[li] = surf;
                 out->log_t[li] = (double)t;
+                if (out->log_point) {
+                    /* the hit point before the offset (no hit: t = BIG, logged as is) */
+                    out->log_point[li * 3] = (double)(origin[0] + t * dir[0]);
+                    out->log_point[li * 3 + 1] = (double)(origin[1] + t * dir[1]);
+                    out->log_point[li * 3 + 2] = (double)(origin[2] + t * dir[2]);
+                }
             }
             if (surf < 0) lc->escaped++;
             if (surf < 0 || t <= p->listener_radius || accum_distance > p->max_path_len) {

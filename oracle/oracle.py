@@ -79,7 +79,7 @@ def _structs(real):
     class Out(ctypes.Structure):
         _fields_ = [
             ("echogram", _P), ("ambisonic", _P), ("hit_counts", _P), ("contrib_counts", _P),
-            ("surface_energy", _P), ("log_prim", _P), ("log_surface", _P), ("log_t", _P),
+            ("surface_energy", _P), ("log_prim", _P), ("log_surface", _P), ("log_t", _P), ("log_point", _P),
             ("visual_points", _P), ("visual_surfaces", _P), ("visual_rays", _P), ("visual_orders", _P),
             ("visual_distances", _P), ("visual_gains", _P),
             ("actual_bounces", _I64), ("active_count", _I64), ("bounce_queries", _I64), ("shadow_queries", _I64),
@@ -108,6 +108,9 @@ def load() -> ctypes.CDLL:
             fn = getattr(lib, f"ao_closest_hit_{suffix}")
             fn.restype = _I32
             fn.argtypes = [ctypes.POINTER(Scene), _P, _P, ctypes.POINTER(_I32), ctypes.POINTER(_I32), _P, _P]
+            fn = getattr(lib, f"ao_diffuse_direction_{suffix}")
+            fn.restype = None
+            fn.argtypes = [_P, _P, _P]
         lib.ao_build_bvh.restype = _I32
         lib.ao_build_bvh.argtypes = [_P, _P, _I64, _I64, _P, _P, _P, _P, _P, _P, _P]
         lib.ao_philox4x32_10.restype = None
@@ -143,6 +146,17 @@ def counter_bounce(seed: int, ray: int, bounce: int, num_samples: int) -> tuple[
     j = np.zeros(1, dtype=np.int32)
     load().ao_counter_bounce(int(seed), int(ray), int(bounce), int(num_samples), _p(u), _p(j))
     return int(u[0]), int(j[0])
+
+
+def diffuse_direction(sample, normal, precision: str = "float64") -> np.ndarray:
+    """The oracle's diffuse continuation: bank sample ``sample`` turned into
+    the frame of the oriented normal ``normal`` (steam_rt.py:4564-4582)."""
+    real = np.float64 if precision == "float64" else np.float32
+    s = np.ascontiguousarray(sample, dtype=real)
+    n = np.ascontiguousarray(normal, dtype=real)
+    out = np.zeros(3, dtype=real)
+    getattr(load(), "ao_diffuse_direction_" + ("f64" if precision == "float64" else "f32"))(_p(s), _p(n), _p(out))
+    return out.astype(np.float64)
 
 
 def build_bvh(lower: np.ndarray, upper: np.ndarray, leaf_size: int = 4) -> dict[str, Any]:
@@ -293,10 +307,12 @@ def trace(scene: OracleScene, *, source, listener, num_rays: int, num_bounces: i
     log_prim = np.full((max(log_rays, 0), max(log_bounces, 0)), -2, dtype=np.int32)
     log_surf = np.full_like(log_prim, -2)
     log_t = np.zeros(log_prim.shape)
+    log_point = np.zeros(log_prim.shape + (3,))
     out.echogram, out.ambisonic = _p(echogram), _p(amb)
     out.hit_counts, out.contrib_counts, out.surface_energy = _p(hits), _p(contribs), _p(senergy)
     if log_rays > 0 and log_bounces > 0:
         out.log_prim, out.log_surface, out.log_t = _p(log_prim), _p(log_surf), _p(log_t)
+        out.log_point = _p(log_point)
     V = max(int(visual_limit), 0)
     p.visual_limit, p.visual_stride = V, max(int(visual_stride), 1)
     vis = {
@@ -328,7 +344,7 @@ def trace(scene: OracleScene, *, source, listener, num_rays: int, num_bounces: i
             "contributions", "escaped")},
     }
     if log_rays > 0 and log_bounces > 0:
-        res.update(log_prim=log_prim, log_surface=log_surf, log_t=log_t)
+        res.update(log_prim=log_prim, log_surface=log_surf, log_t=log_t, log_point=log_point)
     if V > 0:
         res.update(vis)
     return res

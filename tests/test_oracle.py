@@ -541,3 +541,159 @@ def test_visual_strongest_contribution_is_the_largest_bin_of_the_ray():
         assert 1 <= order <= 40
         checked += 1
     assert checked >= 8
+
+
+# ---------------------------------------------------------------------------
+# diffuse frame (steam_rt.py:4564-4582), weighted-dipole gain (4803-4804),
+# source-sphere kill (4757-4760)
+# ---------------------------------------------------------------------------
+
+_SAMPLE = np.asarray([0.3, 0.5, 0.0])
+_SAMPLE[2] = math.sqrt(1.0 - 0.3 ** 2 - 0.5 ** 2)
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-15), ("float32", 2e-7)])
+def test_diffuse_frame_known_answers(precision, tol):
+    s = _SAMPLE
+    # n = +z: helper x, t = x cross n = (0,-1,0), b = n cross t = (1,0,0)
+    np.testing.assert_allclose(O.diffuse_direction(s, (0, 0, 1), precision), (s[1], -s[0], s[2]), atol=tol)
+    # n = -z: t = x cross (-z) = (0,1,0), b = (-z) cross (0,1,0) = (1,0,0)
+    np.testing.assert_allclose(O.diffuse_direction(s, (0, 0, -1), precision), (s[1], s[0], -s[2]), atol=tol)
+    # |n_x| >= 0.9 switches the helper to y: n = +x gives t = (0,0,-1), b = (0,1,0)
+    np.testing.assert_allclose(O.diffuse_direction(s, (1, 0, 0), precision), (s[2], s[1], -s[0]), atol=tol)
+    # n = -x: t = y cross (-x) = (0,0,1), b = (-x) cross z = (0,1,0)
+    np.testing.assert_allclose(O.diffuse_direction(s, (-1, 0, 0), precision), (-s[2], s[1], s[0]), atol=tol)
+    # at the switch: n_x = 0.9 uses y, n_x = 0.89 uses x (both with n_z = 0)
+    ny = math.sqrt(1.0 - 0.81)
+    want = s[0] * np.asarray([0, 0, -1.0]) + s[1] * np.asarray([-ny, 0.9, 0]) + s[2] * np.asarray([0.9, ny, 0])
+    np.testing.assert_allclose(O.diffuse_direction(s, (0.9, ny, 0), precision), want, atol=4 * tol)
+    ny = math.sqrt(1.0 - 0.89 ** 2)
+    want = s[0] * np.asarray([0, 0, 1.0]) + s[1] * np.asarray([ny, -0.89, 0]) + s[2] * np.asarray([0.89, ny, 0])
+    np.testing.assert_allclose(O.diffuse_direction(s, (0.89, ny, 0), precision), want, atol=4 * tol)
+
+
+def test_diffuse_frame_is_right_handed_orthonormal():
+    # unit samples pick out the frame's columns: (t, b, n) orthonormal, t x b = n
+    rng = np.random.default_rng(3)
+    normals = list(rng.normal(size=(200, 3))) + [np.asarray(v, float) for v in
+                                                 ((0.9, 0.4358898943540674, 0), (-0.95, 0.2, 0.1), (0, 0, 1))]
+    for n in normals:
+        n = n / np.linalg.norm(n)
+        t = O.diffuse_direction((1, 0, 0), n)
+        b = O.diffuse_direction((0, 1, 0), n)
+        nn = O.diffuse_direction((0, 0, 1), n)
+        np.testing.assert_allclose(nn, n, atol=1e-15)
+        for v in (t, b):
+            assert abs(np.linalg.norm(v) - 1.0) < 1e-14
+        assert abs(t @ b) < 1e-14 and abs(t @ n) < 1e-14 and abs(b @ n) < 1e-14
+        np.testing.assert_allclose(np.cross(t, b), n, atol=1e-14)
+
+
+def _box_exit(origin, direction, size=(5.0, 4.0, 3.0)):
+    """Distance to the first wall of an axis-aligned room [0, size] from inside."""
+    ts = []
+    for a in range(3):
+        d = direction[a]
+        if abs(d) > 1e-15:
+            ts.append(((size[a] if d > 0 else 0.0) - origin[a]) / d)
+    return min(ts)
+
+
+@pytest.mark.parametrize("precision,tol", [("float64", 1e-12), ("float32", 2e-5)])
+def test_diffuse_continuation_single_ray(precision, tol):
+    """A ray forced diffuse at its first hit (U = 0 < sigma) leaves along the
+    bank sample turned into the hit's frame; its second segment is the
+    distance to the next wall along that hand-computed direction."""
+    mat = Material(absorption=(0.1,) * 6, scattering=0.5)
+    room = RoomScene(RoomSpec(corners=((0, 0), (5, 0), (5, 4), (0, 4)), height=3.0, wall_material=mat,
+                              floor_material=mat, ceiling_material=mat))
+    arrays = room.kernel_arrays()
+    bank = np.asarray([_SAMPLE])
+    s = _SAMPLE
+    lis = np.asarray((2.5, 2.0, 1.5))
+    cases = [
+        # straight down onto the floor: oriented normal +z, direction (s1, -s0, s2)
+        ((0.0, 0.0, -1.0), 1.5, np.asarray((2.5, 2.0, 0.01)), np.asarray((s[1], -s[0], s[2]))),
+        # onto the wall x = 0: oriented normal +x, direction (s2, s1, -s0)
+        ((-1.0, 0.0, 0.0), 2.5, np.asarray((0.01, 2.0, 1.5)), np.asarray((s[2], s[1], -s[0]))),
+    ]
+    for mesh in (None, tessellate(arrays)):
+        scene = O.OracleScene(arrays, mesh, precision)
+        for d0, t0, h0, d1 in cases:
+            res = O.trace(scene, source=(4.5, 3.5, 2.5), listener=lis, num_rays=1, num_bounces=2, num_bins=100,
+                          bin_dur=1e-3, diffuse_bank=bank, directions=np.asarray([d0]),
+                          diffuse_random=np.zeros((2, 1)), diffuse_indices=np.zeros((2, 1), dtype=np.int32),
+                          log_rays=1, log_bounces=2)
+            assert res["log_t"][0, 0] == pytest.approx(t0, abs=tol)
+            t1 = _box_exit(h0, d1)
+            assert res["log_t"][0, 1] == pytest.approx(t1, abs=tol * 10)
+            np.testing.assert_allclose(res["log_point"][0, 1], h0 + t1 * d1, atol=tol * 20)
+
+
+@pytest.mark.parametrize("precision,rtol", [("float64", 1e-12), ("float32", 1e-5)])
+def test_weighted_dipole_gain_closed_form(precision, rtol):
+    """Single floor hit with the source straight above (connection direction
+    +z): the dipole changes each band energy by |(1 - w) + w cos|^p with
+    cos = -(f . +z), so the ratio to an omni trace is exact."""
+    room = RoomScene(RoomSpec(corners=((0, 0), (5, 0), (5, 4), (0, 4)), height=3.0, wall_material=_MAT))
+    arrays = room.kernel_arrays()
+    src, lis = np.asarray((2.5, 2.0, 2.8)), np.asarray((2.5, 2.0, 1.5))
+    bank = sampling.diffuse_sample_bank(128)
+    kw = dict(source=src, listener=lis, num_rays=1, num_bounces=1, num_bins=1000, bin_dur=1e-3, diffuse_bank=bank,
+              directions=np.asarray([[0.0, 0.0, -1.0]]), diffuse_random=np.full((1, 1), 0.99),
+              diffuse_indices=np.zeros((1, 1), dtype=np.int32))
+    for mesh in (None, tessellate(arrays)):
+        scene = O.OracleScene(arrays, mesh, precision)
+        omni = O.trace(scene, **kw)["echogram"].sum(axis=1).astype(np.float64)
+        assert np.all(omni > 0)
+        for forward, w, p, gain in (
+                ((0, 0, -1), 0.37, 2.0, 1.0),                        # cos = 1
+                ((1, 0, 0), 0.37, 2.0, 0.63 ** 2),                   # cos = 0
+                ((0, 0, 1), 0.37, 2.0, 0.26 ** 2),                   # cos = -1
+                ((0, 0, 1), 0.8, 1.5, 0.6 ** 1.5),                   # |1 - 1.6|: the magnitude is taken
+                ((0, 0.6, -0.8), 0.5, 3.0, (0.5 + 0.5 * 0.8) ** 3)):  # cos = 0.8
+            dip = O.trace(scene, source_forward=forward, dipole_weight=w, dipole_power=p, **kw)
+            np.testing.assert_allclose(dip["echogram"].sum(axis=1) / omni, gain, rtol=rtol)
+
+
+@pytest.mark.parametrize("precision", ["float64", "float32"])
+def test_source_sphere_kill(precision):
+    """All-specular shoebox.  Listener (1, 1, 1.5), first ray towards the wall
+    x = 0 at (0, 1.5, 1.5); the mirror segment leaves the offset hit point
+    (0.01, 1.5, 1.5) along (2, 1, 0)/sqrt(5) and next meets the wall x = 5 at
+    t = 4.99 sqrt(5) / 2.  A source 2.5 m along that segment ends the ray at
+    bounce 1 (actual bounces 2, not active; only the bounce-0 hit is counted),
+    a source 0.3 m off the segment does not (the ray is alive after its two
+    bounces and both hits count), and the source sphere is not tested on the
+    first segment (bounce 0)."""
+    mat = Material(absorption=(0.1,) * 6, scattering=0.0)
+    room = RoomScene(RoomSpec(corners=((0, 0), (5, 0), (5, 4), (0, 4)), height=3.0, wall_material=mat,
+                              floor_material=mat, ceiling_material=mat))
+    arrays = room.kernel_arrays()
+    lis = np.asarray((1.0, 1.0, 1.5))
+    d0 = np.asarray([-1.0, 0.5, 0.0]) / math.sqrt(1.25)
+    h0 = np.asarray((0.01, 1.5, 1.5))
+    d1 = np.asarray([2.0, 1.0, 0.0]) / math.sqrt(5.0)
+    on_path = h0 + 2.5 * d1
+    bank = sampling.diffuse_sample_bank(128)
+    for mesh in (None, tessellate(arrays)):
+        scene = O.OracleScene(arrays, mesh, precision)
+
+        def run(source, bounces):
+            return O.trace(scene, source=source, listener=lis, num_rays=1, num_bounces=bounces, num_bins=100,
+                           bin_dur=1e-3, diffuse_bank=bank, directions=np.asarray([d0]),
+                           diffuse_random=np.full((bounces, 1), 0.5),
+                           diffuse_indices=np.zeros((bounces, 1), dtype=np.int32), log_rays=1, log_bounces=bounces)
+
+        killed = run(on_path, 2)
+        assert killed["log_surface"][0, 1] == 1
+        assert killed["log_t"][0, 1] == pytest.approx(4.99 * math.sqrt(5.0) / 2.0, rel=1e-5)
+        assert killed["actual_bounces"] == 2 and killed["active_count"] == 0
+        assert killed["hit_counts"].tolist() == [0, 0, 0, 1, 0, 0]
+        assert killed["counters"]["contributions"] == 1
+        free = run(on_path + np.asarray((0.0, 0.0, 0.3)), 2)
+        assert free["actual_bounces"] == 2 and free["active_count"] == 1
+        assert free["hit_counts"].tolist() == [0, 1, 0, 1, 0, 0]
+        # a source on the first segment: that segment is not tested (bounce 0)
+        first = run(lis + 0.6 * d0, 1)
+        assert first["actual_bounces"] == 1 and first["active_count"] == 1
