@@ -25,6 +25,7 @@ import numpy as np
 from . import _lib
 from .hip_backend import DeviceScene, TraceBuffers, launch_trace, make_params
 from .sampling import diffuse_sample_bank
+from .steam_rt import histogram_bins
 
 C = 343.0
 
@@ -50,11 +51,15 @@ class TraceRequest:
     dipole_weight: float = 0.0
     dipole_power: float = 1.0
     render_ambisonics: bool = False
+    num_diffuse_samples: int = 128  # rows of the diffuse sample bank (rt_num_diffuse_samples)
+    use_bvh: bool = True
+    log_rays: int = 0     # hit log (triangle, t) of rays [0, log_rays) for the first log_bounces bounces
+    log_bounces: int = 0
     tag: Any = None
 
     @property
     def num_bins(self) -> int:
-        return max(1, int(np.ceil(self.duration_s / self.bin_dur - 1e-9)))
+        return histogram_bins(self.duration_s, self.bin_dur)
 
     @property
     def num_sources(self) -> int:
@@ -62,14 +67,16 @@ class TraceRequest:
 
     def params(self) -> _lib.TraceParams:
         return make_params(
-            num_rays=self.num_rays, num_bounces=self.num_bounces, num_bins=self.num_bins, num_diffuse_samples=128,
-            num_sources=self.num_sources, rng_mode=_lib.AA_RNG_COUNTER, use_bvh=True,
+            num_rays=self.num_rays, num_bounces=self.num_bounces, num_bins=self.num_bins,
+            num_diffuse_samples=self.num_diffuse_samples, num_sources=self.num_sources,
+            rng_mode=_lib.AA_RNG_COUNTER, use_bvh=self.use_bvh,
             render_ambisonics=self.render_ambisonics, seed=self.seed, listener=self.listener,
             source_forward=self.source_forward, bin_dur=self.bin_dur, speed_of_sound=self.c,
             max_path_len=self.duration_s * self.c, listener_radius=self.listener_radius,
             source_radius=self.source_radius, irradiance_min_distance=self.irradiance_min_distance,
             specular_exponent=self.specular_exponent, dipole_weight=self.dipole_weight,
-            dipole_power=self.dipole_power)
+            dipole_power=self.dipole_power, log_rays=min(self.log_rays, self.num_rays),
+            log_bounces=self.log_bounces if self.log_rays > 0 else 0)
 
     def sources4(self) -> np.ndarray:
         src = np.asarray(self.sources, dtype=np.float64).reshape(-1, 3)
@@ -134,10 +141,11 @@ def _plan(requests: Sequence[TraceRequest]) -> _Layout:
 class BatchResult:
     """Device outputs of one submission; ``wait`` copies them back."""
 
-    def __init__(self, requests, layout, arenas, event, device, host=None) -> None:
+    def __init__(self, requests, layout, arenas, event, device, host=None, logs=None) -> None:
         self.requests = requests
         self.layout = layout
         self.arenas = arenas
+        self.logs = logs or {}  # request index -> (triangle, t) device tensors
         self.event = event
         self.device = device
         self.host_arenas = host  # pinned copies queued on the submission stream (or None)
@@ -174,6 +182,9 @@ class BatchResult:
                 }
                 if "ambisonic" in slot:
                     res["ambisonic"] = take(f32, "ambisonic", (Ks, 6, 4, K))
+                if len(out) in self.logs:
+                    tri, t = self.logs[len(out)]
+                    res["log_triangle"], res["log_t"] = tri.cpu().numpy(), t.cpu().numpy()
                 out.append(res)
             self._host = out
         return self._host
@@ -192,8 +203,14 @@ class BatchEngine:
         handle = ctypes.c_void_p()
         _lib.check(lib.aa_batch_create(self.device, self.max_traces, ctypes.byref(handle)))
         self._handle = handle
-        self.bank = torch.from_numpy(np.ascontiguousarray(diffuse_sample_bank(128), dtype=np.float32)).to(
-            f"cuda:{self.device}")
+        self._banks: dict[int, Any] = {}
+
+    def bank(self, rows: int):
+        """The diffuse sample bank of ``rows`` rows on this device (uploaded once)."""
+        if rows not in self._banks:
+            self._banks[rows] = self.torch.from_numpy(
+                np.ascontiguousarray(diffuse_sample_bank(rows), dtype=np.float32)).to(f"cuda:{self.device}")
+        return self._banks[rows]
 
     def close(self) -> None:
         if self._handle is not None:
@@ -233,13 +250,19 @@ class BatchEngine:
         src = torch.from_numpy(src_host).to(dev, non_blocking=False)
         table = (_lib.BatchTrace * len(reqs))()
         fp, ip, lp, sp = f32.data_ptr(), i32.data_ptr(), i64.data_ptr(), src.data_ptr()
-        bank = self.bank.data_ptr()
+        logs = {}
+        for i, r in enumerate(reqs):
+            if r.log_rays > 0 and r.log_bounces > 0:
+                # bounces a ray never reaches keep -2 / 0
+                shape = (min(r.log_rays, r.num_rays), r.log_bounces)
+                logs[i] = (torch.full(shape, -2, dtype=torch.int32, device=dev),
+                           torch.zeros(shape, dtype=torch.float32, device=dev))
         for i, (r, slot) in enumerate(zip(reqs, lay.slots)):
             t = table[i]
             t.scene = r.scene.handle
             t.params = r.params()
             t.d_sources = sp + slot["sources"][0] * 16
-            t.d_diffuse_bank = bank
+            t.d_diffuse_bank = self.bank(r.num_diffuse_samples).data_ptr()
             o = t.outputs
             o.d_echogram = fp + slot["echogram"][0] * 4
             o.d_ambisonic = (fp + slot["ambisonic"][0] * 4) if "ambisonic" in slot else 0
@@ -249,6 +272,8 @@ class BatchEngine:
             o.d_actual_bounces = ip + slot["actual_bounces"][0] * 4
             o.d_active_count = ip + slot["active_count"][0] * 4
             o.d_counters = lp + slot["counters"][0] * 8
+            if i in logs:
+                o.d_log_triangle, o.d_log_t = logs[i][0].data_ptr(), logs[i][1].data_ptr()
         s = stream if stream is not None else torch.cuda.current_stream(self.device)
         ts = tuning.as_struct()
         _lib.check(_lib.load().aa_batch_submit(self._handle, table, len(reqs), ctypes.byref(ts),
@@ -261,7 +286,7 @@ class BatchEngine:
                     h.copy_(a, non_blocking=True)
         event = torch.cuda.Event()
         event.record(s)
-        return BatchResult(reqs, lay, (f32, i32, i64, src), event, self.device, host)
+        return BatchResult(reqs, lay, (f32, i32, i64, src), event, self.device, host, logs)
 
 
 def trace_sequential(requests: Sequence[TraceRequest], stream=None,
@@ -277,7 +302,8 @@ def trace_sequential(requests: Sequence[TraceRequest], stream=None,
     t0 = time.perf_counter()
     for r in requests:
         dev = f"cuda:{r.scene.device}"
-        bank = torch.from_numpy(np.ascontiguousarray(diffuse_sample_bank(128), dtype=np.float32)).to(dev)
+        bank = torch.from_numpy(np.ascontiguousarray(diffuse_sample_bank(r.num_diffuse_samples),
+                                                     dtype=np.float32)).to(dev)
         src = torch.from_numpy(r.sources4().astype(np.float32)).to(dev)
         buf = TraceBuffers.allocate(r.scene.device, r.num_sources, r.num_bins, r.scene.num_surfaces,
                                     r.render_ambisonics)

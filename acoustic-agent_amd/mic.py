@@ -1,79 +1,92 @@
-"""Receiver models and their channel positions.
+"""Receiver layouts: where each channel of a receiver listens.
 
-Mirrors acoustic_agent/mic.py:11-80: ``microphone_array(kind, **kwargs)``
-describes a mono, HRTF, linear or circular receiver (same kinds, defaults and
-clamping: linear 2..64 channels at 0.005..2 m spacing, default 4 at 8 cm;
-circular 3..128 channels of radius 0.005..5 m, default 8 at 12 cm; offsets
-rounded to 1 um), and ``channel_positions(receiver, model)`` gives the
-listener position of every channel (one position for mono and HRTF).  Each
-position is one listener-origin trace, as the reference traces each array
-channel on its own (engine.py:64-73).
+The tracer is listener-centric, so a receiver enters it only as the list of
+listener positions it needs: one per channel of a linear or circular array
+(the reference traces each array channel on its own, engine.py:64-73), one
+for a mono or HRTF receiver (both ears share the centre; binaural rendering
+happens after tracing and is not part of this package).
+
+Receiver models are dictionaries (``type``, ``orientation_deg``,
+``channels`` = ``[{"id", "offset"}]``) so that models built by the
+reference's ``microphone_array`` (acoustic_agent/mic.py:11-35) can be passed
+in unchanged.  ``microphone_array`` builds them from ``LAYOUTS``: per kind,
+the parameter that sets the channel count and the one that sets the size,
+each with its default and clamping range (the reference's values), and the
+offset generator.  Offsets lie in the horizontal plane, rotated by the
+orientation, and are rounded to 1 um.
 """
 from __future__ import annotations
 
 import math
-from typing import Any, Mapping, Sequence
+from dataclasses import dataclass
+from typing import Any, Callable, Mapping, Sequence
 
 import numpy as np
 
-SUPPORTED_ARRAYS = {"mono", "hrtf", "linear", "linear_array", "circular", "circular_array"}
+
+def _linear_offsets(count: int, spacing: float, yaw: float) -> np.ndarray:
+    # channels centred on the receiver along the orientation axis
+    along = (np.arange(count) - 0.5 * (count - 1)) * spacing
+    return np.stack([along * math.cos(yaw), along * math.sin(yaw), np.zeros(count)], axis=1)
 
 
-def _linear_channels(count: int, spacing_m: float, orientation_deg: float) -> list[dict[str, Any]]:
-    a = math.radians(orientation_deg)
-    ux, uy = math.cos(a), math.sin(a)
-    mid = 0.5 * (count - 1)
-    return [{"id": f"M{i + 1}",
-             "offset": [round(ux * (i - mid) * spacing_m, 6), round(uy * (i - mid) * spacing_m, 6), 0.0]}
-            for i in range(count)]
+def _circular_offsets(count: int, radius: float, yaw: float) -> np.ndarray:
+    # channel 0 on the orientation axis, then counter-clockwise
+    angle = yaw + 2.0 * math.pi * np.arange(count) / count
+    return np.stack([radius * np.cos(angle), radius * np.sin(angle), np.zeros(count)], axis=1)
 
 
-def _circular_channels(count: int, radius_m: float, orientation_deg: float) -> list[dict[str, Any]]:
-    a0 = math.radians(orientation_deg)
-    out = []
-    for i in range(count):
-        a = a0 + 2.0 * math.pi * i / count
-        out.append({"id": f"M{i + 1}", "offset": [round(math.cos(a) * radius_m, 6), round(math.sin(a) * radius_m, 6), 0.0]})
-    return out
+@dataclass(frozen=True)
+class Layout:
+    kind: str
+    count: tuple[str, int, int, int]           # keyword, default, min, max
+    size: tuple[str, float, float, float]      # keyword, default, min, max
+    offsets: Callable[[int, float, float], np.ndarray]
+
+
+LAYOUTS: dict[str, Layout] = {
+    "linear_array": Layout("linear_array", ("count", 4, 2, 64), ("spacing_m", 0.08, 0.005, 2.0), _linear_offsets),
+    "circular_array": Layout("circular_array", ("count", 8, 3, 128), ("radius_m", 0.12, 0.005, 5.0),
+                             _circular_offsets),
+}
+# receivers with one listening position: channel ids at the centre
+CENTRED: dict[str, tuple[str, ...]] = {"mono": ("M",), "hrtf": ("L", "R")}
+ALIASES = {"linear": "linear_array", "circular": "circular_array"}
 
 
 def microphone_array(kind: str = "mono", **kwargs: Any) -> dict[str, Any]:
-    """Receiver model dictionary (mic.py:11-35)."""
-    k = str(kind).lower()
-    if k not in SUPPORTED_ARRAYS:
+    """Receiver model dictionary of ``kind`` (mono, hrtf, linear[_array],
+    circular[_array]); array parameters are clamped to their ranges."""
+    name = ALIASES.get(str(kind).lower(), str(kind).lower())
+    yaw_deg = float(kwargs.get("orientation_deg", 0.0))
+    if name in CENTRED:
+        channels = [{"id": cid, "offset": [0.0, 0.0, 0.0]} for cid in CENTRED[name]]
+        return {"type": name, "orientation_deg": yaw_deg, "channels": channels}
+    layout = LAYOUTS.get(name)
+    if layout is None:
         raise ValueError(f"unsupported microphone model: {kind!r}")
-    orientation = float(kwargs.get("orientation_deg", 0.0))
-    if k == "hrtf":
-        return {
-            "type": "hrtf",
-            "orientation_deg": orientation,
-            "interpolation": str(kwargs.get("interpolation", "bilinear")),
-            "spatial_blend": float(np.clip(float(kwargs.get("spatial_blend", 1.0)), 0.0, 1.0)),
-            "loudness_normalization": str(kwargs.get("loudness_normalization", "energy")),
-            "sofa_path": kwargs.get("sofa_path"),
-            "channels": [{"id": "L", "offset": [0.0, 0.0, 0.0]}, {"id": "R", "offset": [0.0, 0.0, 0.0]}],
-        }
-    if k in ("linear", "linear_array"):
-        count = int(np.clip(int(kwargs.get("count", 4)), 2, 64))
-        spacing = float(np.clip(float(kwargs.get("spacing_m", 0.08)), 0.005, 2.0))
-        return {"type": "linear_array", "orientation_deg": orientation,
-                "channels": _linear_channels(count, spacing, orientation)}
-    if k in ("circular", "circular_array"):
-        count = int(np.clip(int(kwargs.get("count", 8)), 3, 128))
-        radius = float(np.clip(float(kwargs.get("radius_m", 0.12)), 0.005, 5.0))
-        return {"type": "circular_array", "orientation_deg": orientation,
-                "channels": _circular_channels(count, radius, orientation)}
-    return {"type": "mono", "orientation_deg": orientation, "channels": [{"id": "M", "offset": [0.0, 0.0, 0.0]}]}
+    key, default, lo, hi = layout.count
+    count = min(hi, max(lo, int(kwargs.get(key, default))))
+    key, default, flo, fhi = layout.size
+    size = min(fhi, max(flo, float(kwargs.get(key, default))))
+    offsets = np.round(layout.offsets(count, size, math.radians(yaw_deg)), 6) + 0.0  # + 0.0: no negative zeros
+    channels = [{"id": f"M{i + 1}", "offset": [float(v) for v in row]} for i, row in enumerate(offsets)]
+    return {"type": name, "orientation_deg": yaw_deg, "channels": channels}
 
 
 def channel_positions(receiver: Sequence[float], model: Mapping[str, Any] | None) -> list[tuple[float, float, float]]:
-    """Listener position of every channel of ``model`` at ``receiver`` (mic.py:38-49)."""
-    m = model or microphone_array("mono")
-    centre = np.asarray(receiver, dtype=float)[:3]
-    if str(m.get("type")) not in ("linear_array", "circular_array"):
+    """Listener position of every traced channel of ``model`` placed at
+    ``receiver`` (the reference's mic.py:38-49): centre + offset for array
+    channels, the centre alone for any other receiver."""
+    centre = np.asarray(receiver, dtype=np.float64)[:3]
+    if model is None or str(model.get("type")) not in LAYOUTS:
         return [tuple(float(v) for v in centre)]
-    out = []
-    for ch in m.get("channels", []):
-        p = centre + np.asarray(ch.get("offset", (0.0, 0.0, 0.0)), dtype=float)[:3]
-        out.append(tuple(float(v) for v in p))
-    return out
+    return [tuple(float(v) for v in centre + np.asarray(ch.get("offset", (0.0, 0.0, 0.0)), dtype=np.float64)[:3])
+            for ch in model.get("channels", [])]
+
+
+def circular_array(centre: Sequence[float], count: int = 4, radius: float = 0.05,
+                   orientation_deg: float = 0.0) -> np.ndarray:
+    """[count, 3] channel positions of a circular array at ``centre``."""
+    model = microphone_array("circular", count=count, radius_m=radius, orientation_deg=orientation_deg)
+    return np.asarray(channel_positions(centre, model))

@@ -58,46 +58,25 @@ def _apply_biquad(values: np.ndarray, b0: float, b1: float, b2: float, a1: float
     return lfilter([b0, b1, b2], [1.0, a1, a2], x).astype(np.float32)
 
 
-def _band_limit(signal: np.ndarray, lo: float, hi: float, fs: int, *, mode: str = "bandpass") -> np.ndarray:
-    """RBJ biquad filters (steam_rt.py:4148-4198): ``lowpass`` is an 8th-order
-    Butterworth at ``hi`` (four sections), ``highpass`` a Q=0.707 section at
-    ``lo``, ``bandpass`` one constant-peak section between ``lo`` and ``hi``."""
+def _butterworth_lowpass(signal: np.ndarray, cutoff_hz: float, fs: int) -> np.ndarray:
+    """8th-order Butterworth low-pass at ``cutoff_hz`` (the crossover filter of
+    steam_rt.py:4148-4172): four RBJ low-pass biquads with section Qs
+    1 / (2 cos((2k + 1) pi / 16)), the cutoff clamped below Nyquist."""
     values = np.asarray(signal, dtype=np.float64).reshape(-1)
     if values.size == 0:
         return values.astype(np.float32)
     rate = max(1, int(fs))
-    nyquist = 0.5 * rate
-    if mode == "lowpass":
-        cutoff = float(np.clip(float(hi), 1e-3, nyquist * 0.999))
-        w0 = 2.0 * math.pi * cutoff / rate
-        cw0, sw0 = math.cos(w0), math.sin(w0)
-        out = values
-        order = 8
-        for section in range(order // 2):
-            q = 1.0 / (2.0 * math.cos((2.0 * section + 1.0) * math.pi / (2.0 * order)))
-            alpha = sw0 / (2.0 * q)
-            a0 = 1.0 + alpha
-            b0 = ((1.0 - cw0) / 2.0) / a0
-            out = _apply_biquad(out, b0, (1.0 - cw0) / a0, b0, (-2.0 * cw0) / a0, (1.0 - alpha) / a0)
-        return out
-    if mode == "highpass":
-        cutoff = float(np.clip(float(lo), 1e-3, nyquist * 0.999))
-        w0 = 2.0 * math.pi * cutoff / rate
-        cw0, sw0 = math.cos(w0), math.sin(w0)
-        alpha = sw0 / (2.0 * 0.707)
-        a0 = 1.0 + alpha
-        b0 = ((1.0 + cw0) / 2.0) / a0
-        return _apply_biquad(values, b0, (-(1.0 + cw0)) / a0, b0, (-2.0 * cw0) / a0, (1.0 - alpha) / a0)
-    if mode != "bandpass":
-        raise ValueError(f"unknown filter mode {mode!r}")
-    lower = float(np.clip(float(lo), 1e-3, nyquist * 0.998))
-    upper = float(np.clip(float(hi), lower + 1e-3, nyquist * 0.999))
-    centre = math.sqrt(max(lower * upper, 1e-12))
-    w0 = 2.0 * math.pi * centre / rate
+    cutoff = float(np.clip(float(cutoff_hz), 1e-3, 0.5 * rate * 0.999))
+    w0 = 2.0 * math.pi * cutoff / rate
     cw0, sw0 = math.cos(w0), math.sin(w0)
-    alpha = sw0 * ((upper - lower) / centre) / 2.0
-    a0 = 1.0 + alpha
-    return _apply_biquad(values, alpha / a0, 0.0, -alpha / a0, (-2.0 * cw0) / a0, (1.0 - alpha) / a0)
+    out = values
+    for section in range(4):
+        q = 1.0 / (2.0 * math.cos((2.0 * section + 1.0) * math.pi / 16.0))
+        alpha = sw0 / (2.0 * q)
+        a0 = 1.0 + alpha
+        b0 = ((1.0 - cw0) / 2.0) / a0
+        out = _apply_biquad(out, b0, (1.0 - cw0) / a0, b0, (-2.0 * cw0) / a0, (1.0 - alpha) / a0)
+    return out
 
 
 def bandlimit_band_signals(signals: np.ndarray, fs: int) -> np.ndarray:
@@ -108,12 +87,11 @@ def bandlimit_band_signals(signals: np.ndarray, fs: int) -> np.ndarray:
         raise ValueError(f"expected {_NUM_BANDS} band signals, got shape {values.shape}")
     crossovers = [edge[1] for edge in _band_edges(fs)[:-1]]
     out = np.zeros_like(values, dtype=np.float32)
-    out[0] = _band_limit(values[0], 0.0, crossovers[0], fs, mode="lowpass")
+    out[0] = _butterworth_lowpass(values[0], crossovers[0], fs)
     for b in range(1, _NUM_BANDS - 1):
-        upper = _band_limit(values[b], 0.0, crossovers[b], fs, mode="lowpass")
-        lower = _band_limit(values[b], 0.0, crossovers[b - 1], fs, mode="lowpass")
-        out[b] = upper - lower
-    out[-1] = values[-1] - _band_limit(values[-1], 0.0, crossovers[-1], fs, mode="lowpass")
+        out[b] = _butterworth_lowpass(values[b], crossovers[b], fs) - _butterworth_lowpass(values[b], crossovers[b - 1],
+                                                                                           fs)
+    out[-1] = values[-1] - _butterworth_lowpass(values[-1], crossovers[-1], fs)
     return out
 
 

@@ -19,6 +19,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from .mic import circular_array
 from .scene import BoxSpec, Material, RoomScene, RoomSpec, TriangleMesh, WallSpec, point_in_polygon, tessellate, \
     tessellate_to_count
 
@@ -148,17 +149,6 @@ def random_scene(seed: int, target_triangles: int) -> Workload:
     scene = l_plan(seed, num_boxes=num_boxes)
     mesh, edge = tessellate_to_count(scene.kernel_arrays(), target_triangles)
     return Workload(scene, mesh, np.asarray(L_SOURCE), np.asarray(L_LISTENER), seed, edge)
-
-
-def circular_array(center, count: int = 4, radius: float = 0.05, orientation_deg: float = 0.0) -> np.ndarray:
-    """Channel positions of a circular microphone array (mic.py:38-49, 65-77)."""
-    c = np.asarray(center, dtype=np.float64)
-    start = math.radians(orientation_deg)
-    out = []
-    for i in range(count):
-        a = start + 2.0 * math.pi * i / count
-        out.append(c + np.asarray([round(math.cos(a) * radius, 6), round(math.sin(a) * radius, 6), 0.0]))
-    return np.asarray(out)
 
 
 def catmull_rom(points: np.ndarray, samples: int) -> np.ndarray:

@@ -50,6 +50,11 @@ def visual_candidate_plan(config: SimConfig, num_rays: int) -> tuple[int, int]:
     return limit, max(1, int(num_rays) // max(limit, 1))
 
 
+def histogram_bins(duration_s: float, bin_dur: float) -> int:
+    """Bins of the time histogram: ceil(duration / bin), at least one (steam_rt.py:1947-1949)."""
+    return max(1, int(math.ceil(float(duration_s) / float(bin_dur))))
+
+
 def _normalize_rt_accelerator(value: str) -> str:
     accelerator = str(value).strip().lower()
     if accelerator not in {"auto", "numba", "cuda", "hip"}:
@@ -120,7 +125,7 @@ def _trace_energy_field_hip(
     num_bounces = int(config.rt_num_bounces)
     bin_dur = float(config.rt_bin_duration_s)
     duration = max(float(config.rt_duration_s), float(config.duration_s))
-    num_bins = max(1, int(math.ceil(duration / bin_dur)))
+    num_bins = histogram_bins(duration, bin_dur)
     bank = diffuse_sample_bank(config.rt_num_diffuse_samples)
     rng = str(config.rt_rng).lower()
     if rng == "reference":

@@ -26,8 +26,12 @@ from .hip_backend import DeviceScene
 
 
 def device_scene(w: scenes.Workload, device: int = 0) -> DeviceScene:
+    """Upload ``w``'s triangles; the scene keeps ``w`` as ``workload`` (its
+    host description, e.g. for a CPU check of the same scene)."""
     a = w.scene.kernel_arrays()
-    return DeviceScene(w.mesh, np.asarray(a["reflection"]), np.asarray(a["scattering"]), device=device)
+    ds = DeviceScene(w.mesh, np.asarray(a["reflection"]), np.asarray(a["scattering"]), device=device)
+    ds.workload = w
+    return ds
 
 
 def log_uniform_triangles(rng: np.random.Generator, lo: int = 12, hi: int = 50_000) -> int:

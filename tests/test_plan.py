@@ -90,3 +90,28 @@ def test_motion_frames_with_an_array_receiver():
         motion_traces("scene", [], cfg)
     with pytest.raises(ValueError):
         motion_traces("scene", [{"source": (0, 0, 0)}], cfg)
+
+
+def test_planned_traces_carry_the_whole_configuration():
+    from acoustic_agent_amd.steam_rt import histogram_bins
+
+    cfg = SimConfig(rt_num_rays=500, rt_num_bounces=7, duration_s=0.5, rt_duration_s=0.9, rt_bin_duration_s=0.03,
+                    rt_num_diffuse_samples=64, rt_specular_exponent=50.0, rt_source_radius=0.2, seed=3)
+    model = {"type": "weighted_dipole", "dipole_weight": 0.37, "dipole_power": 2.0, "orientation_deg": 90.0}
+    reqs = channel_traces("scene", (1, 1, 1), (2, 2, 1.5), microphone_array("linear", count=2), cfg,
+                          source_model=model)
+    # 0.9 / 0.03 is 30.000000000000004 in binary64: 31 bins, as trace_energy_field gives
+    assert histogram_bins(0.9, 0.03) == 31
+    for r in reqs:
+        assert r.num_bins == 31 and r.num_diffuse_samples == 64 and r.render_ambisonics
+        assert r.dipole_weight == 0.37 and r.dipole_power == 2.0
+        np.testing.assert_allclose(r.source_forward, (0.0, 1.0, 0.0), atol=1e-15)
+        assert r.specular_exponent == 50.0 and r.source_radius == 0.2
+        p = r.params()
+        assert p.num_bins == 31 and p.num_diffuse_samples == 64 and p.render_ambisonics == 1
+        assert abs(p.dipole_weight - 0.37) < 1e-7 and p.dipole_power == 2.0
+    plan = motion_traces("scene", [{"source": (1, 1, 1), "receiver": (2, 2, 1.5)}], cfg, source_model="cardioid",
+                         render_ambisonics=False)
+    assert plan.traces[0].dipole_weight == 0.5 and not plan.traces[0].render_ambisonics
+    with pytest.raises(ValueError, match="counter"):
+        channel_traces("scene", (1, 1, 1), (2, 2, 1.5), None, SimConfig(rt_rng="reference"))
