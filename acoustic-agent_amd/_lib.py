@@ -18,7 +18,6 @@ AA_ERR_INVALID = -1
 AA_ERR_DEVICE = -2
 AA_ERR_MEMORY = -3
 AA_ERR_LIMIT = -4
-AA_ERR_KERNEL = -5
 
 AA_RNG_ARRAYS = 0
 AA_RNG_COUNTER = 1
@@ -78,7 +77,6 @@ class TraceParams(ctypes.Structure):
         ("dipole_power", ctypes.c_float),
         ("block_size", ctypes.c_int32),
         ("lds_histogram", ctypes.c_int32),
-        ("lds_nodes", ctypes.c_int32),
         ("workgroups_per_cu", ctypes.c_int32),
         ("sources_per_lane", ctypes.c_int32),
         ("log_rays", ctypes.c_int32),
@@ -100,6 +98,7 @@ class TraceOutputs(ctypes.Structure):
         ("d_counters", ctypes.c_void_p),
         ("d_log_triangle", ctypes.c_void_p),
         ("d_log_t", ctypes.c_void_p),
+        ("d_log_point", ctypes.c_void_p),
         ("d_visual_points", ctypes.c_void_p),
         ("d_visual_surfaces", ctypes.c_void_p),
         ("d_visual_rays", ctypes.c_void_p),
@@ -126,8 +125,6 @@ class BatchTuning(ctypes.Structure):
     _fields_ = [
         ("block_size", ctypes.c_int32),
         ("packet_rays", ctypes.c_int32),
-        ("lds_nodes", ctypes.c_int32),
-        ("lds_histogram", ctypes.c_int32),
         ("workgroups_per_cu", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
     ]

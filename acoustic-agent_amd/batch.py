@@ -16,6 +16,7 @@ the batched path is measured against.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import time
 from dataclasses import dataclass, field
 from typing import Any, Sequence
@@ -53,7 +54,7 @@ class TraceRequest:
     render_ambisonics: bool = False
     num_diffuse_samples: int = 128  # rows of the diffuse sample bank (rt_num_diffuse_samples)
     use_bvh: bool = True
-    log_rays: int = 0     # hit log (triangle, t) of rays [0, log_rays) for the first log_bounces bounces
+    log_rays: int = 0     # hit log (triangle, t, point) of rays [0, log_rays), first log_bounces bounces
     log_bounces: int = 0
     tag: Any = None
 
@@ -93,16 +94,15 @@ class TraceRequest:
 class BatchTuning:
     block_size: int = 256
     packet_rays: int = 0  # 0: about two work items per resident wave
-    lds_nodes: int = 0  # ignored by the library (nodes are read through L1/L2); kept for the C struct
     workgroups_per_cu: int = 0
     use_graph: bool = False
 
     def as_struct(self) -> _lib.BatchTuning:
-        return _lib.BatchTuning(int(self.block_size), int(self.packet_rays), int(self.lds_nodes), 0,
-                                int(self.workgroups_per_cu), 1 if self.use_graph else 0)
+        return _lib.BatchTuning(int(self.block_size), int(self.packet_rays), int(self.workgroups_per_cu),
+                                1 if self.use_graph else 0)
 
     def key(self) -> tuple:
-        return (self.block_size, self.packet_rays, self.lds_nodes, self.workgroups_per_cu, self.use_graph)
+        return (self.block_size, self.packet_rays, self.workgroups_per_cu, self.use_graph)
 
 
 @dataclass
@@ -145,7 +145,7 @@ class BatchResult:
         self.requests = requests
         self.layout = layout
         self.arenas = arenas
-        self.logs = logs or {}  # request index -> (triangle, t) device tensors
+        self.logs = logs or {}  # request index -> (triangle, t, point) device tensors
         self.event = event
         self.device = device
         self.host_arenas = host  # pinned copies queued on the submission stream (or None)
@@ -183,8 +183,8 @@ class BatchResult:
                 if "ambisonic" in slot:
                     res["ambisonic"] = take(f32, "ambisonic", (Ks, 6, 4, K))
                 if len(out) in self.logs:
-                    tri, t = self.logs[len(out)]
-                    res["log_triangle"], res["log_t"] = tri.cpu().numpy(), t.cpu().numpy()
+                    res["log_triangle"], res["log_t"], res["log_point"] = (x.cpu().numpy()
+                                                                           for x in self.logs[len(out)])
                 out.append(res)
             self._host = out
         return self._host
@@ -256,7 +256,8 @@ class BatchEngine:
                 # bounces a ray never reaches keep -2 / 0
                 shape = (min(r.log_rays, r.num_rays), r.log_bounces)
                 logs[i] = (torch.full(shape, -2, dtype=torch.int32, device=dev),
-                           torch.zeros(shape, dtype=torch.float32, device=dev))
+                           torch.zeros(shape, dtype=torch.float32, device=dev),
+                           torch.zeros(shape + (3,), dtype=torch.float32, device=dev))
         for i, (r, slot) in enumerate(zip(reqs, lay.slots)):
             t = table[i]
             t.scene = r.scene.handle
@@ -273,7 +274,7 @@ class BatchEngine:
             o.d_active_count = ip + slot["active_count"][0] * 4
             o.d_counters = lp + slot["counters"][0] * 8
             if i in logs:
-                o.d_log_triangle, o.d_log_t = logs[i][0].data_ptr(), logs[i][1].data_ptr()
+                o.d_log_triangle, o.d_log_t, o.d_log_point = (x.data_ptr() for x in logs[i])
         s = stream if stream is not None else torch.cuda.current_stream(self.device)
         ts = tuning.as_struct()
         _lib.check(_lib.load().aa_batch_submit(self._handle, table, len(reqs), ctypes.byref(ts),
@@ -294,19 +295,29 @@ def trace_sequential(requests: Sequence[TraceRequest], stream=None,
     """One launch per trace, synchronised and copied back per call.
 
     ``plain`` uses the one-thread-per-ray kernel of the reference GPU backend
-    (``aa_trace_plain``), else the persistent single-trace kernel.
+    (``aa_trace_plain``), else the persistent single-trace kernel.  Host
+    scenes are uploaded (once per content) on device 0 for the call and
+    released after it.
     """
     import torch
 
     out = []
+    uploaded: dict[int, DeviceScene] = {}  # host scenes (HostScene) uploaded for this call
     t0 = time.perf_counter()
     for r in requests:
+        if not isinstance(r.scene, DeviceScene):
+            key = r.scene.content_hash
+            if key not in uploaded:
+                uploaded[key] = r.scene.upload(0)
+            r = dataclasses.replace(r, scene=uploaded[key])
         dev = f"cuda:{r.scene.device}"
         bank = torch.from_numpy(np.ascontiguousarray(diffuse_sample_bank(r.num_diffuse_samples),
                                                      dtype=np.float32)).to(dev)
         src = torch.from_numpy(r.sources4().astype(np.float32)).to(dev)
+        logged = r.log_rays > 0 and r.log_bounces > 0
         buf = TraceBuffers.allocate(r.scene.device, r.num_sources, r.num_bins, r.scene.num_surfaces,
-                                    r.render_ambisonics)
+                                    r.render_ambisonics, log_rays=min(r.log_rays, r.num_rays) if logged else 0,
+                                    log_bounces=r.log_bounces if logged else 0)
         s = stream if stream is not None else torch.cuda.current_stream(r.scene.device)
         launch_trace(r.scene, r.params(), src, bank, buf, stream=s, plain=plain)
         s.synchronize()
@@ -322,4 +333,11 @@ def trace_sequential(requests: Sequence[TraceRequest], stream=None,
         })
         if r.render_ambisonics:
             out[-1]["ambisonic"] = buf.ambisonic.cpu().numpy()
-    return out, time.perf_counter() - t0
+        if logged:
+            out[-1]["log_triangle"] = buf.log_triangle.cpu().numpy()
+            out[-1]["log_t"] = buf.log_t.cpu().numpy()
+            out[-1]["log_point"] = buf.log_point.cpu().numpy()
+    elapsed = time.perf_counter() - t0
+    for scene in uploaded.values():
+        scene.close()
+    return out, elapsed

@@ -4,12 +4,11 @@
 // The reference runs one trace per call and fans calls out over a thread pool
 // (acoustic_agent/batch.py:139-184, api.py:477-639); here a batch is a table
 // of trace descriptors in device memory and a queue of work items, each item
-// a packet of rays of one trace.  A grid sized to fill the GPU pulls items
-// with one atomic per item until the queue is empty (every workgroup reaches
-// that exit), re-stages the top BVH nodes in LDS when it moves to another
-// scene, and traces the packet with the per-ray routine of aa_device.h (so a
-// trace gives the same result alone, in a batch, or through aa_trace, up to
-// the order of float atomic sums).
+// a packet of rays of one trace.  A grid sized to fill the GPU pulls items,
+// one wave at a time with one atomic per item, until the queue is empty
+// (every wave reaches that exit), and traces the packet with the per-ray
+// routine of aa_device.h (so a trace gives the same result alone, in a batch,
+// or through aa_trace, up to the order of float atomic sums).
 //
 // A submission is: copy the descriptor table from pinned host memory, zero
 // every trace's outputs with a kernel, run the persistent kernel.  With
@@ -42,7 +41,6 @@ struct BatchTrace {
     SceneView sv;
     RayParams p;
     Outputs out;
-    int* error_flag;
     int first_packet;  // prefix sum of packets over the table
     int num_packets;
     int lds_surf;      // single source and S small enough for LDS tallies
@@ -126,7 +124,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
         const SceneView sv = uniform_copy(&bt.sv);
         const RayParams p = uniform_copy(&bt.p);
         const Outputs out = uniform_copy(&bt.out);
-        int* const error_flag = uniform_copy(&bt.error_flag);
         const int first_packet = __builtin_amdgcn_readfirstlane(bt.first_packet);
         const bool lds_surf = __builtin_amdgcn_readfirstlane(bt.lds_surf) != 0;
         if (lds_surf) {
@@ -149,7 +146,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
         lds.contrib = lds_surf ? my_contrib : nullptr;
         lds.senergy = lds_surf ? my_energy : nullptr;
         LaneStats st;
-        trace_rays_refill<WG, false, false>(sv, p, out, lds, &s_cursor[wave], begin, count, st, error_flag);
+        trace_rays_refill<WG, false, false>(sv, p, out, lds, &s_cursor[wave], begin, count, st);
         flush_stats(st, p, out);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -332,7 +329,6 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
         d.p = aa::make_ray_params(&t.params, t.d_sources, t.d_directions, t.d_diffuse_bank, t.d_diffuse_random,
                                   t.d_diffuse_indices);
         d.out = aa::make_outputs(&t.outputs, &t.params);
-        d.error_flag = t.scene->d_error;
         const int64_t trace_tasks = (int64_t)d.p.num_rays * d.p.groups;
         if (trace_tasks > (int64_t)1 << 30) return fail(AA_ERR_LIMIT, "trace %d has too many lane tasks", i);
         d.num_packets = (int)((trace_tasks + packet_rays - 1) / packet_rays);

@@ -80,6 +80,7 @@ struct Outputs {
     unsigned long long* counters;
     int* log_tri;
     float* log_t;
+    float* log_point;  // [rays, bounces, 3] or null
     int log_rays, log_bounces;
     // visual path candidates (null / 0 when off)
     float* vis_points;  // [V,B,3]
@@ -503,8 +504,7 @@ __device__ __forceinline__ void test_leaf(const SceneView& sv, int start, int co
 // hit; shadow queries beyond the source and stop at the first occluder.
 template <int WG>
 __device__ __forceinline__ void traversal_step(const SceneView& sv, unsigned short* stk,
-                                               Query& q, bool shadow, unsigned& node_cnt, unsigned& tri_cnt,
-                                               int* error_flag) {
+                                               Query& q, bool shadow, unsigned& node_cnt, unsigned& tri_cnt) {
     ++node_cnt;
     float4 v[7];
     load_node(sv, q.node, v);
@@ -555,13 +555,11 @@ __device__ __forceinline__ void traversal_step(const SceneView& sv, unsigned sho
 #undef AA_CSWAP
 #pragma unroll
     for (int c = 3; c >= 1; --c) {
-        if (key[c] < INFINITY) {
-            if (q.sp < kStack) {
-                stk[q.sp * WG] = (unsigned short)idx[c];
-                ++q.sp;
-            } else {
-                atomicOr(error_flag, 1);
-            }
+        // the bound is never reached (aa_scene_create caps 3 (depth - 1) at
+        // kStack); it only keeps the LDS accesses in range
+        if (key[c] < INFINITY && q.sp < kStack) {
+            stk[q.sp * WG] = (unsigned short)idx[c];
+            ++q.sp;
         }
     }
     if (key[0] < INFINITY) {
@@ -816,8 +814,14 @@ __device__ __forceinline__ bool close_bounce(const SceneView& sv, const RayParam
     const int bounce = s.bounce;
     const Closest h = q.h;
     if (out.log_tri != nullptr && s.group == 0 && s.ray < out.log_rays && bounce < out.log_bounces) {
-        out.log_tri[(size_t)s.ray * out.log_bounces + bounce] = h.tri;
-        out.log_t[(size_t)s.ray * out.log_bounces + bounce] = h.t;
+        const size_t li = (size_t)s.ray * out.log_bounces + bounce;
+        out.log_tri[li] = h.tri;
+        out.log_t[li] = h.t;
+        if (out.log_point != nullptr) {
+            out.log_point[li * 3] = s.ox + h.t * s.dx;
+            out.log_point[li * 3 + 1] = s.oy + h.t * s.dy;
+            out.log_point[li * 3 + 2] = s.oz + h.t * s.dz;
+        }
     }
     if (h.surf < 0) ++st.escaped;
     if (h.surf < 0 || h.t <= p.listener_radius || s.accum > p.max_path_len) {
@@ -1002,7 +1006,7 @@ constexpr int kStepsPerRound = 8;
 // another lane's ray; idle lanes take parked connections first, then new rays.
 template <int WG, bool LDS_HIST, bool VISUAL = true>
 __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds,
-                                  unsigned* cursor, int first, int count, LaneStats& st, int* error_flag) {
+                                  unsigned* cursor, int first, int count, LaneStats& st) {
     unsigned short* stk = lds.stack + threadIdx.x;
     // the wave's index, provably uniform (readfirstlane), so that its queue
     // address is a scalar
@@ -1123,7 +1127,7 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
             } else {
 #pragma unroll 1
                 for (int k = 0; k < kStepsPerRound && q.node >= 0; ++k)
-                    traversal_step<WG>(sv, stk, q, shadow, nodes, tris, error_flag);
+                    traversal_step<WG>(sv, stk, q, shadow, nodes, tris);
             }
             if (shadow) {
                 st.ns += nodes;
@@ -1237,6 +1241,7 @@ inline Outputs make_outputs(const aa_trace_outputs* o, const aa_trace_params* p)
     out.log_bounces = log ? p->log_bounces : 0;
     out.log_tri = log ? o->d_log_triangle : nullptr;
     out.log_t = log ? o->d_log_t : nullptr;
+    out.log_point = log ? o->d_log_point : nullptr;
     const bool vis = p->visual_limit > 0;
     out.vis_limit = vis ? p->visual_limit : 0;
     out.vis_stride = vis ? p->visual_stride : 1;

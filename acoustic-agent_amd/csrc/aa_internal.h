@@ -97,7 +97,6 @@ struct aa_scene_s {
     float* d_normals = nullptr;    // [T, 4]
     float* d_reflection = nullptr; // [S, 6]
     float* d_scattering = nullptr; // [S]
-    int* d_error = nullptr;
     uint64_t device_bytes = 0;
     uint64_t content_hash = 0;
 };

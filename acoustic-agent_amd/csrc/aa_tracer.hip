@@ -60,7 +60,6 @@ struct TraceArgs {
     SceneView sv;
     RayParams p;
     Outputs out;
-    int* error_flag;
     int lds_surf;  // per-surface tallies privatised in LDS (single source)
 };
 
@@ -114,7 +113,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void tr
     // persistent lanes: each wave claims rays from the trace's cursor as its
     // lanes' rays end
     unsigned* cursor = reinterpret_cast<unsigned*>(a.out.counters + AA_CNT_RAYS_CLAIMED);
-    trace_rays_refill<WG, LDS_HIST>(a.sv, a.p, a.out, lds, cursor, 0, a.p.num_rays * a.p.groups, st, a.error_flag);
+    trace_rays_refill<WG, LDS_HIST>(a.sv, a.p, a.out, lds, cursor, 0, a.p.num_rays * a.p.groups, st);
     flush_stats(st, a.p, a.out);
     __syncthreads();
     if (LDS_HIST) {
@@ -324,7 +323,6 @@ void free_scene(aa_scene_s* s) {
     (void)hipFree(s->d_normals);
     (void)hipFree(s->d_reflection);
     (void)hipFree(s->d_scattering);
-    (void)hipFree(s->d_error);
     delete s;
 }
 
@@ -559,7 +557,6 @@ int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tr
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_normals, nrm_bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_reflection, refl_bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&s->d_scattering, scat_bytes);
-    if (e == hipSuccess) e = hipMalloc((void**)&s->d_error, sizeof(int));
     if (e == hipSuccess) e = hipMemcpy(s->d_nodes, s->wide.nodes.data(), node_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_tris, tris.data(), tri_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -568,12 +565,11 @@ int aa_scene_create(int32_t device, const float* tri_vertices, const int32_t* tr
     if (e == hipSuccess) e = hipMemcpy(s->d_normals, normals.data(), nrm_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_reflection, reflection, refl_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(s->d_scattering, scattering, scat_bytes, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(s->d_error, 0, sizeof(int));
     if (e != hipSuccess) {
         free_scene(s);
         return fail(AA_ERR_MEMORY, "scene upload failed: %s", hipGetErrorString(e));
     }
-    s->device_bytes = node_bytes + 2 * tri_bytes + plain_node_bytes + nrm_bytes + refl_bytes + scat_bytes + sizeof(int);
+    s->device_bytes = node_bytes + 2 * tri_bytes + plain_node_bytes + nrm_bytes + refl_bytes + scat_bytes;
     *out = s;
     return AA_OK;
 }
@@ -666,6 +662,16 @@ int aa::validate_trace(const aa_scene_s* scene, const aa_trace_params* p, const 
         return fail(AA_ERR_INVALID, "hit log needs d_log_triangle and d_log_t");
     if ((int64_t)p->num_sources * AA_NUM_BANDS * p->num_bins > (int64_t)1 << 30)
         return fail(AA_ERR_LIMIT, "histogram too large");
+    {
+        // lane tasks = rays x source groups (the grouping of make_ray_params);
+        // kernels index tasks with 32-bit ints
+        const int64_t per_lane = p->sources_per_lane > 0 ? std::min(p->sources_per_lane, p->num_sources)
+                                                         : std::min(8, p->num_sources);
+        const int64_t groups = (p->num_sources + per_lane - 1) / per_lane;
+        if ((int64_t)p->num_rays * groups > (int64_t)1 << 30)
+            return fail(AA_ERR_LIMIT, "num_rays x source groups = %lld lane tasks exceeds 2^30",
+                        (long long)p->num_rays * groups);
+    }
     if (p->visual_limit < 0) return fail(AA_ERR_INVALID, "visual_limit must be >= 0");
     if (p->visual_limit > 0) {
         if (p->num_sources != 1) return fail(AA_ERR_INVALID, "visual candidates need a single-source trace");
@@ -752,7 +758,6 @@ extern "C" int aa_trace(aa_scene_t scene, const aa_trace_params* p, const float*
     a.sv = aa::make_scene_view(scene, p->use_bvh != 0);
     a.p = aa::make_ray_params(p, d_sources, d_directions, d_diffuse_bank, d_diffuse_random, d_diffuse_indices);
     a.out = aa::make_outputs(o, p);
-    a.error_flag = scene->d_error;
     a.lds_surf = (Ks == 1 && stack_bytes + (lds_hist ? hist_bytes : 0) + surf_bytes <=
                   std::max(lds_resident, stack_bytes + (lds_hist ? hist_bytes : 0))) ? 1 : 0;
     const size_t lds = stack_bytes + (lds_hist ? hist_bytes : 0) + (a.lds_surf ? surf_bytes : 0);

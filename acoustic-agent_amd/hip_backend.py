@@ -134,6 +134,55 @@ class DeviceScene:
             pass
 
 
+class HostScene:
+    """A triangle scene on the host, identified by its content hash.
+
+    The hash is ``aa_scene_hash`` over the same float32/int32 inputs
+    ``aa_scene_create`` takes, so a host scene and its resident device copy
+    share one key (a serving cache uploads a host scene once and reuses the
+    device copy for every request with the same content).  ``workload``
+    optionally keeps the description the scene was compiled from."""
+
+    # upper bound of device bytes per triangle before upload: at most
+    # max(1, T - 1) wide nodes of 128 B, two triangle copies of 48 B, 2T - 1
+    # binary nodes of 32 B and a 16 B normal
+    BOUND_BYTES_PER_TRIANGLE = 128 + 96 + 64 + 16
+
+    def __init__(self, mesh: TriangleMesh, reflection: np.ndarray, scattering: np.ndarray, *, leaf_size: int = 4,
+                 workload: Any = None) -> None:
+        self.mesh = mesh
+        self.reflection = np.ascontiguousarray(reflection, dtype=np.float32)
+        self.scattering = np.ascontiguousarray(scattering, dtype=np.float32)
+        if self.reflection.shape != (mesh.num_surfaces, _lib.AA_NUM_BANDS) or \
+                self.scattering.shape != (mesh.num_surfaces,):
+            raise ValueError("reflection must be [S,6] and scattering [S] for the mesh's S surfaces")
+        self.leaf_size = int(leaf_size)
+        self.workload = workload
+        v = np.ascontiguousarray(mesh.vertices, dtype=np.float32)
+        s = np.ascontiguousarray(mesh.surface, dtype=np.int32)
+        n = np.ascontiguousarray(mesh.normals, dtype=np.float32)
+        h = ctypes.c_uint64(0)
+        _lib.check(_lib.load().aa_scene_hash(_ptr(v), _ptr(s), _ptr(n), int(v.shape[0]), _ptr(self.reflection),
+                                             _ptr(self.scattering), int(mesh.num_surfaces), self.leaf_size,
+                                             ctypes.byref(h)))
+        self.content_hash = int(h.value)
+        self.info = {"num_triangles": int(v.shape[0]), "num_surfaces": int(mesh.num_surfaces),
+                     "content_hash": self.content_hash}
+
+    @property
+    def num_surfaces(self) -> int:
+        return int(self.mesh.num_surfaces)
+
+    @property
+    def bound_device_bytes(self) -> int:
+        return self.info["num_triangles"] * self.BOUND_BYTES_PER_TRIANGLE + 128 + self.num_surfaces * 28 + 4
+
+    def upload(self, device: int = 0) -> "DeviceScene":
+        ds = DeviceScene(self.mesh, self.reflection, self.scattering, device=device, leaf_size=self.leaf_size)
+        ds.workload = self.workload
+        return ds
+
+
 def scene_from_arrays(arrays: Mapping[str, Any], *, device: int = 0, max_edge: float | None = None,
                       leaf_size: int = 4) -> DeviceScene:
     mesh = tessellate(arrays, max_edge)
@@ -252,6 +301,7 @@ class TraceBuffers:
     counters: Any
     log_triangle: Any
     log_t: Any
+    log_point: Any = None
     visual: tuple | None = None  # points, surfaces, rays, orders, distances, gains (visual candidates)
 
     @classmethod
@@ -277,6 +327,7 @@ class TraceBuffers:
             # bounces a ray never reaches keep -2 / 0
             log_triangle=torch.full((log_rays, log_bounces), -2, dtype=i32, device=dev) if log_rays > 0 else None,
             log_t=torch.zeros((log_rays, log_bounces), dtype=f32, device=dev) if log_rays > 0 else None,
+            log_point=torch.zeros((log_rays, log_bounces, 3), dtype=f32, device=dev) if log_rays > 0 else None,
             visual=visual,
         )
 
@@ -287,7 +338,7 @@ class TraceBuffers:
         vis = [p(t) for t in self.visual] if self.visual is not None else [ctypes.c_void_p(0)] * 6
         return _lib.TraceOutputs(p(self.echogram), p(self.ambisonic), p(self.hit_counts), p(self.contrib_counts),
                                  p(self.surface_energy), p(self.actual_bounces), p(self.active_count), p(self.counters),
-                                 p(self.log_triangle), p(self.log_t), *vis)
+                                 p(self.log_triangle), p(self.log_t), p(self.log_point), *vis)
 
 
 def make_params(*, num_rays: int, num_bounces: int, num_bins: int, num_diffuse_samples: int, num_sources: int,
@@ -322,7 +373,6 @@ def make_params(*, num_rays: int, num_bounces: int, num_bins: int, num_diffuse_s
     p.dipole_power = float(dipole_power)
     p.block_size = int(t.get("block_size", 0))
     p.lds_histogram = int(t.get("lds_histogram", -1))
-    p.lds_nodes = int(t.get("lds_nodes", -1))
     p.workgroups_per_cu = int(t.get("workgroups_per_cu", 0))
     p.sources_per_lane = int(t.get("sources_per_lane", 0))
     p.log_rays = int(log_rays)
@@ -499,6 +549,7 @@ def trace_energy_field_hip(
     if buffers.log_triangle is not None:
         result["log_triangle"] = buffers.log_triangle.cpu().numpy()
         result["log_t"] = buffers.log_t.cpu().numpy()
+        result["log_point"] = buffers.log_point.cpu().numpy()
     result["kernel_time_s"] = kernel_time
     result["transfer_time_s"] = upload_time + (time.perf_counter() - download_started)
     info = device_info(device)

@@ -9,9 +9,18 @@
   along a seeded 64-keyframe spline at 0.5-1.5 m/s and a fixed listener; all
   keyframes of a scene share one ray set (the reference reuses config and seed
   for every keyframe, api.py:683-696), so they are one multi-source trace.
-* ``served_stream``: Poisson arrivals of requests drawn from the shoebox,
-  single-scene, mic-array and keyframe mixes, over a pool of scenes whose
-  BVHs stay resident and are reused across requests.
+* ``served_stream``: Poisson arrivals of requests drawn from the four
+  workload kinds, over a pool of host scenes (``HostScene``, keyed by content
+  hash: the serving loop uploads each once and reuses it):
+    - ``shoebox``: the 5 x 4 x 3 m shoebox (12 triangles), 10k rays, 50
+      bounces, 1.0 s;
+    - ``single``: one trace of a scene of the 12-50k triangle mix, 10k-200k
+      rays, 200 bounces, 2.0 s;
+    - ``mic_array``: the same with a 1-8 microphone ring (5 cm), one trace
+      per channel;
+    - ``keyframes``: a furnished L plan (~20k triangles) with its seeded
+      64-keyframe source spline, one 64-source trace of 100k rays, 200
+      bounces, 1.0 s.
 """
 from __future__ import annotations
 
@@ -22,7 +31,7 @@ import numpy as np
 
 from . import scenes
 from .batch import TraceRequest
-from .hip_backend import DeviceScene
+from .hip_backend import DeviceScene, HostScene
 
 
 def device_scene(w: scenes.Workload, device: int = 0) -> DeviceScene:
@@ -32,6 +41,12 @@ def device_scene(w: scenes.Workload, device: int = 0) -> DeviceScene:
     ds = DeviceScene(w.mesh, np.asarray(a["reflection"]), np.asarray(a["scattering"]), device=device)
     ds.workload = w
     return ds
+
+
+def host_scene(w: scenes.Workload) -> HostScene:
+    """``w``'s triangles on the host, keyed by content hash (keeps ``w``)."""
+    a = w.scene.kernel_arrays()
+    return HostScene(w.mesh, np.asarray(a["reflection"]), np.asarray(a["scattering"]), workload=w)
 
 
 def log_uniform_triangles(rng: np.random.Generator, lo: int = 12, hi: int = 50_000) -> int:
@@ -74,43 +89,51 @@ class ServedRequest:
     arrival_s: float
     size_class: str
     traces: list[TraceRequest]
+    kind: str = "single"
+
+
+SERVED_KINDS = ("shoebox", "single", "mic_array", "keyframes")
+SERVED_MIX = (0.3, 0.3, 0.25, 0.15)
 
 
 def served_stream(rate_hz: float, duration_s: float, seed: int = 0, num_scenes: int = 12,
-                  device: int = 0, max_rays: int = 200_000) -> list[ServedRequest]:
+                  num_keyframe_scenes: int = 2, max_rays: int = 200_000) -> list[ServedRequest]:
     """Poisson arrivals at ``rate_hz`` over ``duration_s`` of mixed solves."""
     rng = np.random.default_rng(seed)
+    shoebox = scenes.shoebox_workload(seed=int(rng.integers(0, 1 << 31)))
+    shoebox_scene = host_scene(shoebox)
     pool = []
-    for i in range(num_scenes):
-        tri = log_uniform_triangles(rng, 12, 50_000)
-        w = scenes.random_scene(int(rng.integers(0, 1 << 31)), tri)
-        pool.append((w, device_scene(w, device)))
+    for _ in range(num_scenes):
+        w = scenes.random_scene(int(rng.integers(0, 1 << 31)), log_uniform_triangles(rng, 12, 50_000))
+        pool.append((w, host_scene(w)))
+    moving = []
+    for _ in range(num_keyframe_scenes):
+        w, path = scenes.keyframe_workload(int(rng.integers(0, 1 << 31)), 20_000, 64)
+        moving.append((w, path, host_scene(w)))
     t = 0.0
     out: list[ServedRequest] = []
     while True:
         t += float(rng.exponential(1.0 / rate_hz))
         if t > duration_s:
             break
-        w, ds = pool[int(rng.integers(0, len(pool)))]
-        kind = rng.choice(["shoebox_small", "single", "mic_array", "keyframes"], p=[0.3, 0.3, 0.25, 0.15])
-        rays = int(rng.integers(10_000, max_rays + 1))
+        kind = str(rng.choice(SERVED_KINDS, p=SERVED_MIX))
         req_seed = int(rng.integers(0, 1 << 31))
-        if kind == "shoebox_small":
-            rays = 10_000
-            traces = [TraceRequest(scene=ds, listener=w.listener, sources=w.source[None, :], num_rays=rays,
+        if kind == "shoebox":
+            w, hs = shoebox, shoebox_scene
+            traces = [TraceRequest(scene=hs, listener=w.listener, sources=w.source[None, :], num_rays=10_000,
                                    num_bounces=50, duration_s=1.0, seed=req_seed)]
-        elif kind == "single":
-            traces = [TraceRequest(scene=ds, listener=w.listener, sources=w.source[None, :], num_rays=rays,
-                                   num_bounces=200, duration_s=2.0, seed=req_seed)]
-        elif kind == "mic_array":
-            count = int(rng.integers(1, 9))
-            mics = scenes.circular_array(w.listener, count, 0.05)
-            traces = [TraceRequest(scene=ds, listener=m, sources=w.source[None, :], num_rays=rays, num_bounces=200,
-                                   duration_s=2.0, seed=req_seed) for m in mics]
-        else:
-            path = w.source[None, :] + np.linspace(0.0, 0.5, 16)[:, None] * np.asarray([1.0, 0.3, 0.0])[None, :]
-            traces = [TraceRequest(scene=ds, listener=w.listener, sources=path, num_rays=rays, num_bounces=200,
+        elif kind == "keyframes":
+            w, path, hs = moving[int(rng.integers(0, len(moving)))]
+            traces = [TraceRequest(scene=hs, listener=w.listener, sources=path, num_rays=100_000, num_bounces=200,
                                    duration_s=1.0, seed=req_seed)]
-        size = "small" if w.mesh.num_triangles < 1000 else ("medium" if w.mesh.num_triangles < 10_000 else "large")
-        out.append(ServedRequest(arrival_s=t, size_class=f"{size}", traces=traces))
+        else:
+            w, hs = pool[int(rng.integers(0, len(pool)))]
+            rays = int(rng.integers(10_000, max_rays + 1))
+            count = int(rng.integers(1, 9)) if kind == "mic_array" else 1
+            listeners = scenes.circular_array(w.listener, count, 0.05) if kind == "mic_array" else [w.listener]
+            traces = [TraceRequest(scene=hs, listener=m, sources=w.source[None, :], num_rays=rays, num_bounces=200,
+                                   duration_s=2.0, seed=req_seed) for m in listeners]
+        tri = w.mesh.num_triangles
+        size = "small" if tri < 1000 else ("medium" if tri < 10_000 else "large")
+        out.append(ServedRequest(arrival_s=t, size_class=size, traces=traces, kind=kind))
     return out

@@ -32,8 +32,7 @@ enum {
     AA_ERR_INVALID = -1,  /* bad argument: maps to the reference's ValueError */
     AA_ERR_DEVICE = -2,   /* HIP runtime failure / no device: RuntimeError */
     AA_ERR_MEMORY = -3,   /* allocation failed or over budget */
-    AA_ERR_LIMIT = -4,    /* scene or request exceeds a compiled limit */
-    AA_ERR_KERNEL = -5    /* a kernel reported an internal overflow */
+    AA_ERR_LIMIT = -4     /* scene or request exceeds a compiled limit */
 };
 
 /* RNG streams. */
@@ -139,7 +138,6 @@ typedef struct {
     float dipole_power;
     int32_t block_size;          /* 0: tuned default; else 64/128/256/512 */
     int32_t lds_histogram;       /* -1: tuned default; 0: global atomics; 1: LDS privatised */
-    int32_t lds_nodes;           /* ignored: nodes are read through L1/L2 (field kept for layout) */
     int32_t workgroups_per_cu;   /* persistent grid = CUs x this (0: occupancy limit) */
     int32_t sources_per_lane;    /* multi-source traces: sources sharing one lane's trajectory
                                     (0: 8); the trace runs num_rays x ceil(Ks / this) lane tasks */
@@ -163,6 +161,7 @@ typedef struct {
     int64_t *d_counters;      /* [AA_CNT_COUNT] */
     int32_t *d_log_triangle;  /* [log_rays, log_bounces] hit triangle (-1 none) or NULL */
     float *d_log_t;           /* [log_rays, log_bounces] or NULL */
+    float *d_log_point;       /* [log_rays, log_bounces, 3] hit point o + t d (before the offset), or NULL */
     float *d_visual_points;   /* [V,B,3] offset hit point per bounce (0: none); V = visual_limit */
     int32_t *d_visual_surfaces;   /* [V,B] surface id per bounce (-1: none) */
     int32_t *d_visual_rays;       /* [V] ray index of the candidate (-1: none) */
@@ -193,7 +192,7 @@ int aa_trace_plain(aa_scene_t scene, const aa_trace_params *params, const float 
 
 /* Synchronise `stream` (device errors come back as AA_ERR_DEVICE).  The
  * traversal stack cannot overflow: aa_scene_create rejects trees deeper than
- * the stack holds (AA_ERR_LIMIT). */
+ * the stack holds (AA_ERR_LIMIT), so kernels have no overflow to report. */
 int aa_stream_sync(void *stream);
 
 /* Poll `stream` without blocking: *done = 1 once every call queued on it
@@ -229,8 +228,6 @@ typedef struct {
 typedef struct {
     int32_t block_size;         /* 64, 128, 256 or 512 lanes per workgroup */
     int32_t packet_rays;        /* rays per work item */
-    int32_t lds_nodes;          /* ignored: nodes are read through L1/L2 (field kept for layout) */
-    int32_t lds_histogram;      /* ignored: batch histograms use global atomics (kept for layout) */
     int32_t workgroups_per_cu;  /* persistent grid = CUs x this (0: from LDS use) */
     int32_t use_graph;          /* 1: capture copy + zero + kernel in a hipGraph and replay it */
 } aa_batch_tuning;

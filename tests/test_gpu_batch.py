@@ -61,9 +61,9 @@ def test_batch_equals_single_traces(requests):
 
 
 @pytest.mark.parametrize("tuning", [
-    BatchTuning(block_size=64, packet_rays=256, lds_nodes=0),
-    BatchTuning(block_size=128, packet_rays=4096, lds_nodes=5),
-    BatchTuning(block_size=512, packet_rays=1024, lds_nodes=85, workgroups_per_cu=1),
+    BatchTuning(block_size=64, packet_rays=256),
+    BatchTuning(block_size=128, packet_rays=4096, workgroups_per_cu=2),
+    BatchTuning(block_size=512, packet_rays=1024, workgroups_per_cu=1),
 ])
 def test_batch_tunings_agree(requests, tuning):
     engine = BatchEngine(0, max_traces=64)

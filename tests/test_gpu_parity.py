@@ -99,9 +99,9 @@ def test_bvh_equals_linear_on_gpu(lplan):
 
 
 @pytest.mark.parametrize("tuning", [
-    {"block_size": 64, "lds_histogram": 0, "lds_nodes": 0},
-    {"block_size": 128, "lds_histogram": 1, "lds_nodes": 5},
-    {"block_size": 512, "lds_histogram": 1, "lds_nodes": 21},
+    {"block_size": 64, "lds_histogram": 0},
+    {"block_size": 128, "lds_histogram": 1, "workgroups_per_cu": 2},
+    {"block_size": 512, "lds_histogram": 1, "workgroups_per_cu": 1},
 ])
 def test_launch_shapes_agree(lplan, tuning):
     case = Case(lplan, num_rays=4096, num_bounces=60, num_bins=2000, seed=6, rng="counter")

@@ -1,7 +1,8 @@
 """GPU: the serving loop and the autotuner.
 
-Each served result must be the same alone (one plain launch per trace) and in
-the continuous-batching mix: counts exactly, histograms to rel L1 <= 1e-5.
+Each served result must be the same alone (one persistent single-trace launch
+per trace) and in the continuous-batching mix: counts exactly, histograms to
+rel L1 <= 1e-5.  The HBM budget charges resident scenes and output arenas.
 """
 from __future__ import annotations
 
@@ -11,7 +12,7 @@ import pytest
 from acoustic_agent_amd import workloads
 from acoustic_agent_amd.autotune import BATCH_SPACE, Autotuner
 from acoustic_agent_amd.batch import trace_sequential
-from acoustic_agent_amd.serving import ServingLoop, request_bytes, serve_sequential
+from acoustic_agent_amd.serving import SceneCache, ServingLoop, _host_scenes, request_bytes, serve_sequential
 
 from helpers import rel_l1
 
@@ -42,15 +43,37 @@ def test_served_results_equal_alone(stream):
 
 
 def test_hbm_budget_limits_admission(stream):
-    small = max(request_bytes(r) for r in stream[:6])
-    loop = ServingLoop(0, slots=2, max_batch_traces=32, hbm_budget_bytes=small)
-    done, _ = loop.run(stream[:6], realtime=False)
-    assert len(done) == 6
+    # the largest request with its own scenes (upper bound) is the smallest budget that serves the stream:
+    # scenes are evicted between batches to make room, and results do not change
+    first = stream[:6]
+    alone = max(request_bytes(r) + SceneCache().pending_bytes(_host_scenes(r)) for r in first)
+    loop = ServingLoop(0, slots=2, max_batch_traces=32, hbm_budget_bytes=alone)
+    done, summary = loop.run(first, realtime=False)
     loop.close()
-    loop = ServingLoop(0, slots=1, max_batch_traces=32, hbm_budget_bytes=small - 1)
+    assert len(done) == 6 and summary["peak_charged_bytes"] <= alone
+    distinct = len({h.content_hash for r in first for h in _host_scenes(r)})
+    assert summary["scene_cache"]["misses"] >= distinct
+    roomy = ServingLoop(0, slots=2, max_batch_traces=32)
+    done2, summary2 = roomy.run(first, realtime=False)
+    roomy.close()
+    assert summary2["scene_cache"]["misses"] == distinct and summary2["scene_cache"]["evictions"] == 0
+    by_arrival = {c.request.arrival_s: c for c in done2}
+    for c in done:
+        for a, b in zip(c.results, by_arrival[c.request.arrival_s].results):
+            np.testing.assert_array_equal(a["hit_counts"], b["hit_counts"])
+            assert rel_l1(a["echogram"], b["echogram"]) <= 1e-5
+    loop = ServingLoop(0, slots=1, max_batch_traces=32, hbm_budget_bytes=alone - 1)
     with pytest.raises(ValueError):
-        loop.run(stream[:6], realtime=False)
+        loop.run(first, realtime=False)
     loop.close()
+
+
+def test_scene_bound_covers_the_uploaded_scene(stream):
+    for r in stream[:8]:
+        for h in _host_scenes(r):
+            ds = h.upload(0)
+            assert ds.device_bytes <= h.bound_device_bytes and ds.info["content_hash"] == h.content_hash
+            ds.close()
 
 
 def test_sequential_baseline_reports_latency(stream):
@@ -59,7 +82,11 @@ def test_sequential_baseline_reports_latency(stream):
 
 
 def test_autotuner_picks_from_space_and_caches(stream, tmp_path):
-    reqs = [t for r in stream[:6] for t in r.traces]
+    from acoustic_agent_amd.serving import _resolve
+
+    cache = SceneCache(0)
+    traces = [t for r in stream[:6] for t in r.traces]
+    reqs = _resolve(traces, cache.acquire(t.scene for t in traces))
     path = tmp_path / "tune.json"
     tuner = Autotuner(0, cache_path=path)
     space = {"block_size": (128, 256), "packet_rays": (0, 1024, 4096), "workgroups_per_cu": (0,)}
@@ -68,4 +95,5 @@ def test_autotuner_picks_from_space_and_caches(stream, tmp_path):
     assert len(tuner.log) == 6
     again = Autotuner(0, cache_path=path).tune_batch(reqs, space=space, sample=8, reps=1)
     assert again.key() == best.key()
-    assert set(BATCH_SPACE) == set(space)
+    assert set(BATCH_SPACE) >= set(space)
+    cache.close()

@@ -68,6 +68,29 @@ def test_abi_rejects_bad_inputs_before_touching_the_device():
         _lib.AA_ERR_INVALID
 
 
+def test_abi_caps_lane_tasks_before_touching_the_device():
+    # rays x source groups index 32-bit lane tasks: more than 2^30 is rejected
+    # by validation, before the (stand-in) scene handle is dereferenced
+    lib = _lib.load()
+    dummy = ctypes.create_string_buffer(64)
+    addr = ctypes.cast(dummy, ctypes.c_void_p).value
+    p = hip_backend.make_params(
+        num_rays=40_000_000, num_bounces=10, num_bins=100, num_diffuse_samples=128, num_sources=64,
+        rng_mode=_lib.AA_RNG_COUNTER, use_bvh=True, render_ambisonics=False, seed=1, listener=(0, 0, 0),
+        source_forward=(1, 0, 0), bin_dur=1e-3, speed_of_sound=343.0, max_path_len=100.0, listener_radius=0.1,
+        source_radius=0.1, irradiance_min_distance=1.0, specular_exponent=100.0, dipole_weight=0.0,
+        dipole_power=1.0, tuning={"sources_per_lane": 1})
+    out = _lib.TraceOutputs(*([addr] * 8))
+    for fn in (lib.aa_trace, lib.aa_trace_plain):
+        rc = fn(addr, ctypes.byref(p), addr, None, addr, None, None, ctypes.byref(out), None)
+        assert rc == _lib.AA_ERR_LIMIT and "lane tasks" in lib.aa_last_error().decode()
+    # 64 sources in groups of 8 (the default): 8 groups x 40M rays is within the cap
+    p.num_rays = 134_217_728 + 1
+    p.sources_per_lane = 0
+    assert lib.aa_trace(addr, ctypes.byref(p), addr, None, addr, None, None, ctypes.byref(out), None) == \
+        _lib.AA_ERR_LIMIT
+
+
 # ---------------------------------------------------------------------------
 # dispatch (steam_rt.py:1757-1819 and tests/test_accelerators.py:117-155)
 # ---------------------------------------------------------------------------
