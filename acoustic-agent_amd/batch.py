@@ -96,13 +96,14 @@ class BatchTuning:
     packet_rays: int = 0  # 0: about two work items per resident wave
     workgroups_per_cu: int = 0
     use_graph: bool = False
+    sources_per_lane: int = 0  # multi-source traces: sources per lane task (0: each trace's own setting)
 
     def as_struct(self) -> _lib.BatchTuning:
         return _lib.BatchTuning(int(self.block_size), int(self.packet_rays), int(self.workgroups_per_cu),
                                 1 if self.use_graph else 0)
 
     def key(self) -> tuple:
-        return (self.block_size, self.packet_rays, self.workgroups_per_cu, self.use_graph)
+        return (self.block_size, self.packet_rays, self.workgroups_per_cu, self.use_graph, self.sources_per_lane)
 
 
 @dataclass
@@ -262,6 +263,8 @@ class BatchEngine:
             t = table[i]
             t.scene = r.scene.handle
             t.params = r.params()
+            if tuning.sources_per_lane > 0 and r.num_sources > 1:
+                t.params.sources_per_lane = int(tuning.sources_per_lane)
             t.d_sources = sp + slot["sources"][0] * 16
             t.d_diffuse_bank = self.bank(r.num_diffuse_samples).data_ptr()
             o = t.outputs

@@ -204,6 +204,17 @@ class ServingLoop:
             e.close()
         self.scenes.close()
 
+    def warm(self, stream: Sequence[ServedRequest]) -> int:
+        """Make the scenes of ``stream`` resident ahead of time (as a running
+        server has its working set resident), within the budget; returns
+        the number uploaded."""
+        before = self.scenes.stats["misses"]
+        for r in stream:
+            hosts = _host_scenes(r)
+            if self.scenes.bytes + self.scenes.pending_bytes(hosts) <= self.hbm_budget:
+                self.scenes.release(list(self.scenes.acquire(hosts)))
+        return self.scenes.stats["misses"] - before
+
     def _fits(self, outputs: int, new_scene_bytes: int, keep: set[int]) -> bool:
         """Whether ``outputs`` more output bytes and ``new_scene_bytes`` of
         uploads fit the budget, evicting unused scenes (not in ``keep``) if
