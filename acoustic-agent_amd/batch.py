@@ -227,19 +227,13 @@ class BatchEngine:
         except Exception:
             pass
 
-    def submit(self, requests: Sequence[TraceRequest], tuning: BatchTuning | None = None, stream=None,
-               order: bool = True, copy_back: bool = False) -> BatchResult:
-        """Enqueue one persistent batch on ``stream``.
-
-        ``copy_back`` also queues the copies of every output into pinned host
-        memory on the same stream, so that ``done()`` means the results are
-        on the host and reading them waits on nothing else.
-        """
+    def pack(self, requests: Sequence[TraceRequest], tuning: BatchTuning | None = None, order: bool = True):
+        """The C descriptor table of ``requests`` with freshly allocated output
+        arenas (the caller's outputs of the submission): returns (table,
+        requests in table order, layout, arenas, hit logs)."""
         torch = self.torch
         if not requests:
             raise ValueError("empty batch")
-        if len(requests) > self.max_traces:
-            raise ValueError(f"at most {self.max_traces} traces per submission")
         tuning = tuning or BatchTuning()
         reqs = sorted(requests, key=lambda r: -r.work) if order else list(requests)
         lay = _plan(reqs)
@@ -278,19 +272,37 @@ class BatchEngine:
             o.d_counters = lp + slot["counters"][0] * 8
             if i in logs:
                 o.d_log_triangle, o.d_log_t, o.d_log_point = (x.data_ptr() for x in logs[i])
-        s = stream if stream is not None else torch.cuda.current_stream(self.device)
-        ts = tuning.as_struct()
-        _lib.check(_lib.load().aa_batch_submit(self._handle, table, len(reqs), ctypes.byref(ts),
-                                               ctypes.c_void_p(s.cuda_stream)))
+        return table, reqs, lay, (f32, i32, i64, src), logs
+
+    def _finish(self, reqs, lay, arenas, logs, s, copy_back: bool) -> BatchResult:
+        torch = self.torch
         host = None
         if copy_back:
-            host = tuple(torch.empty(a.shape, dtype=a.dtype, pin_memory=True) for a in (f32, i32, i64))
+            host = tuple(torch.empty(a.shape, dtype=a.dtype, pin_memory=True) for a in arenas[:3])
             with torch.cuda.stream(s):
-                for h, a in zip(host, (f32, i32, i64)):
+                for h, a in zip(host, arenas[:3]):
                     h.copy_(a, non_blocking=True)
         event = torch.cuda.Event()
         event.record(s)
-        return BatchResult(reqs, lay, (f32, i32, i64, src), event, self.device, host, logs)
+        return BatchResult(reqs, lay, arenas, event, self.device, host, logs)
+
+    def submit(self, requests: Sequence[TraceRequest], tuning: BatchTuning | None = None, stream=None,
+               order: bool = True, copy_back: bool = False) -> BatchResult:
+        """Enqueue one persistent batch on ``stream`` (``aa_batch_submit``).
+
+        ``copy_back`` also queues the copies of every output into pinned host
+        memory on the same stream, so that ``done()`` means the results are
+        on the host and reading them waits on nothing else.
+        """
+        if len(requests) > self.max_traces:
+            raise ValueError(f"at most {self.max_traces} traces per submission")
+        tuning = tuning or BatchTuning()
+        table, reqs, lay, arenas, logs = self.pack(requests, tuning, order)
+        s = stream if stream is not None else self.torch.cuda.current_stream(self.device)
+        ts = tuning.as_struct()
+        _lib.check(_lib.load().aa_batch_submit(self._handle, table, len(reqs), ctypes.byref(ts),
+                                               ctypes.c_void_p(s.cuda_stream)))
+        return self._finish(reqs, lay, arenas, logs, s, copy_back)
 
 
 def trace_sequential(requests: Sequence[TraceRequest], stream=None,

@@ -179,11 +179,6 @@ struct aa_batch_s {
 
 namespace {
 
-size_t table_size(int max_traces) {
-    const size_t hdr = (sizeof(aa::BatchHeader) + 255) & ~size_t(255);
-    return hdr + (size_t)max_traces * sizeof(aa::BatchTrace);
-}
-
 template <int WG>
 hipError_t launch_batch(aa::BatchHeader* hdr, const aa::BatchTrace* traces, int grid, size_t lds, int surf_cap,
                         hipStream_t stream) {
@@ -206,6 +201,156 @@ int occupancy_blocks(size_t lds) {
 
 }  // namespace
 
+namespace aa {
+
+// Launch plan of one batch: validated traces, LDS layout, persistent grid
+// and packet size (shared by aa_batch_submit and aa_batch_submit_static).
+struct BatchPlan {
+    int device = 0, wg = 256, waves = 4, surf_cap = 0, per_cu = 1, packet_rays = 0, grid = 1, persistent_grid = 1;
+    size_t lds = 0;
+};
+
+size_t table_bytes(int traces) {
+    const size_t hdr = (sizeof(BatchHeader) + 255) & ~size_t(255);
+    return hdr + (size_t)traces * sizeof(BatchTrace);
+}
+
+int plan_batch(int device, const aa_batch_trace* traces, int32_t count, const aa_batch_tuning* tuning,
+               BatchPlan& plan) {
+    plan.device = device;
+    plan.wg = tuning->block_size > 0 ? tuning->block_size : 256;
+    const int wg = plan.wg;
+    if (wg != 64 && wg != 128 && wg != 256 && wg != 512) return fail(AA_ERR_INVALID, "block_size must be 64..512");
+    if (tuning->packet_rays > (1 << 24)) return fail(AA_ERR_INVALID, "packet_rays out of range");
+    for (int32_t i = 0; i < count; ++i) {
+        const aa_batch_trace& t = traces[i];
+        int rc = validate_trace(t.scene, &t.params, t.d_sources, t.d_directions, t.d_diffuse_bank,
+                                t.d_diffuse_random, t.d_diffuse_indices, &t.outputs);
+        if (rc != AA_OK) return rc;
+        if (t.scene->device != device) return fail(AA_ERR_INVALID, "trace %d: scene on another device", i);
+        if (t.params.visual_limit > 0)
+            return fail(AA_ERR_INVALID, "trace %d: visual candidates are recorded by aa_trace only", i);
+    }
+    if (hipSetDevice(device) != hipSuccess) return fail(AA_ERR_DEVICE, "cannot select device %d", device);
+    const DeviceProps& props = device_props(device);
+    if (!props.valid) return fail(AA_ERR_DEVICE, "cannot query device %d", device);
+
+    // LDS plan shared by every trace of the launch: per-wave surface tallies
+    // (histograms go to global atomics)
+    int max_surf = 0;
+    for (int32_t i = 0; i < count; ++i)
+        if (traces[i].params.num_sources == 1) max_surf = std::max(max_surf, traces[i].scene->num_surfaces);
+    plan.waves = wg / 64;
+    const size_t lds_cap = props.lds_per_block > 0 ? props.lds_per_block : 65536;
+    // surface tallies only while the workgroups the register budget allows
+    // (12 waves per CU) stay resident
+    const size_t lds_resident = (props.lds_per_cu > 0 ? props.lds_per_cu : lds_cap) /
+                                (size_t)std::max(1, 12 * 64 / wg);
+    plan.surf_cap =
+        (max_surf > 0 && private_bytes(wg) + (size_t)plan.waves * max_surf * 12 <= lds_resident) ? max_surf : 0;
+    plan.lds = private_bytes(wg) + (size_t)plan.waves * plan.surf_cap * 12;
+    // the kernel's static LDS (the per-wave cursors) comes on top of the
+    // dynamic request
+    const size_t static_lds = (size_t)plan.waves * sizeof(unsigned);
+    if (plan.lds + static_lds > lds_cap)
+        return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", plan.lds, lds_cap);
+
+    int per_cu = tuning->workgroups_per_cu;
+    if (per_cu <= 0) {
+        switch (wg) {
+            case 64: per_cu = occupancy_blocks<64>(plan.lds); break;
+            case 128: per_cu = occupancy_blocks<128>(plan.lds); break;
+            case 256: per_cu = occupancy_blocks<256>(plan.lds); break;
+            default: per_cu = occupancy_blocks<512>(plan.lds); break;
+        }
+    }
+    plan.per_cu = per_cu;
+    // packet size: the tuning's, or (0) about two work items per resident
+    // wave so that small batches still spread over the whole GPU
+    int64_t tasks = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const aa_batch_trace& t = traces[i];
+        const int g = t.params.sources_per_lane > 0 ? std::min(t.params.sources_per_lane, t.params.num_sources)
+                                                    : std::min(8, t.params.num_sources);
+        tasks += (int64_t)t.params.num_rays * ((t.params.num_sources + g - 1) / g);
+    }
+    int packet_rays = tuning->packet_rays;
+    if (packet_rays <= 0) {
+        const int64_t resident = (int64_t)props.cu * per_cu * plan.waves;
+        const int64_t want = (tasks + 2 * resident - 1) / (2 * resident);
+        packet_rays = (int)std::min<int64_t>(16384, std::max<int64_t>(64, (want + 63) / 64 * 64));
+    }
+    plan.packet_rays = packet_rays;
+    int64_t packets = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const aa_batch_trace& t = traces[i];
+        const int g = t.params.sources_per_lane > 0 ? std::min(t.params.sources_per_lane, t.params.num_sources)
+                                                    : std::min(8, t.params.num_sources);
+        const int64_t trace_tasks = (int64_t)t.params.num_rays * ((t.params.num_sources + g - 1) / g);
+        packets += (trace_tasks + packet_rays - 1) / packet_rays;
+    }
+    if (packets > (int64_t)1 << 30) return fail(AA_ERR_LIMIT, "too many work items (%lld)", (long long)packets);
+    plan.persistent_grid = std::max(1, props.cu * per_cu);
+    plan.grid = (int)std::max<int64_t>(1, std::min<int64_t>(plan.persistent_grid, (packets + plan.waves - 1) / plan.waves));
+    return AA_OK;
+}
+
+// The descriptor table (header, then one BatchTrace per trace) of a planned
+// batch, written to host memory `table` of at least table_bytes(count).
+int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& plan, void* table) {
+    const size_t hdr_bytes = (sizeof(BatchHeader) + 255) & ~size_t(255);
+    BatchHeader* h_hdr = static_cast<BatchHeader*>(table);
+    BatchTrace* h_tr = reinterpret_cast<BatchTrace*>(static_cast<char*>(table) + hdr_bytes);
+    int64_t packets = 0;
+    for (int32_t i = 0; i < count; ++i) {
+        const aa_batch_trace& t = traces[i];
+        BatchTrace& d = h_tr[i];
+        std::memset(&d, 0, sizeof(d));
+        d.sv = make_scene_view(t.scene, t.params.use_bvh != 0);
+        d.p = make_ray_params(&t.params, t.d_sources, t.d_directions, t.d_diffuse_bank, t.d_diffuse_random,
+                              t.d_diffuse_indices);
+        d.out = make_outputs(&t.outputs, &t.params);
+        const int64_t trace_tasks = (int64_t)d.p.num_rays * d.p.groups;
+        if (trace_tasks > (int64_t)1 << 30) return fail(AA_ERR_LIMIT, "trace %d has too many lane tasks", i);
+        d.num_packets = (int)((trace_tasks + plan.packet_rays - 1) / plan.packet_rays);
+        d.first_packet = (int)packets;
+        packets += d.num_packets;
+        d.lds_surf = (t.params.num_sources == 1 && t.scene->num_surfaces <= plan.surf_cap) ? 1 : 0;
+        const OutputExtents ext = output_extents(&t.outputs, &t.params, t.scene->num_surfaces);
+        for (int r = 0; r < 8; ++r) {
+            d.zero_ptr[r] = static_cast<unsigned*>(ext.ptr[r]);
+            d.zero_words[r] = (unsigned)(ext.bytes[r] / 4);
+        }
+    }
+    h_hdr->num_traces = count;
+    h_hdr->total_packets = (int)packets;
+    h_hdr->packet_rays = plan.packet_rays;
+    h_hdr->queue_head = 0u;
+    return AA_OK;
+}
+
+// Copy `copy_bytes` of the host table to `d_table`, zero the outputs of
+// `zero_traces` entries, run the persistent kernel on `grid` workgroups.
+hipError_t enqueue_batch(void* d_table, const void* h_table, size_t copy_bytes, int zero_traces,
+                         const BatchPlan& plan, int grid, hipStream_t s) {
+    const size_t hdr_bytes = (sizeof(BatchHeader) + 255) & ~size_t(255);
+    BatchHeader* d_hdr = static_cast<BatchHeader*>(d_table);
+    const BatchTrace* d_tr = reinterpret_cast<const BatchTrace*>(static_cast<char*>(d_table) + hdr_bytes);
+    hipError_t e = hipMemcpyAsync(d_table, h_table, copy_bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_zero_kernel, dim3(zero_traces), dim3(256), 0, s, d_hdr, d_tr);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    switch (plan.wg) {
+        case 64: return launch_batch<64>(d_hdr, d_tr, grid, plan.lds, plan.surf_cap, s);
+        case 128: return launch_batch<128>(d_hdr, d_tr, grid, plan.lds, plan.surf_cap, s);
+        case 256: return launch_batch<256>(d_hdr, d_tr, grid, plan.lds, plan.surf_cap, s);
+        default: return launch_batch<512>(d_hdr, d_tr, grid, plan.lds, plan.surf_cap, s);
+    }
+}
+
+}  // namespace aa
+
 extern "C" {
 
 int aa_batch_create(int32_t device, int32_t max_traces, aa_batch_t* out) {
@@ -219,7 +364,7 @@ int aa_batch_create(int32_t device, int32_t max_traces, aa_batch_t* out) {
     aa_batch_s* b = new aa_batch_s();
     b->device = device;
     b->max_traces = max_traces;
-    b->table_bytes = table_size(max_traces);
+    b->table_bytes = aa::table_bytes(max_traces);
     hipError_t e = hipHostMalloc(&b->h_table, b->table_bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&b->d_table, b->table_bytes);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->copied, hipEventDisableTiming);
@@ -251,134 +396,28 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
                     void* stream_ptr) {
     if (!b || !traces || !tuning) return fail(AA_ERR_INVALID, "aa_batch_submit: NULL argument");
     if (count < 1 || count > b->max_traces) return fail(AA_ERR_INVALID, "count must be in [1, %d]", b->max_traces);
-    const int wg = tuning->block_size > 0 ? tuning->block_size : 256;
-    if (wg != 64 && wg != 128 && wg != 256 && wg != 512) return fail(AA_ERR_INVALID, "block_size must be 64..512");
-    if (tuning->packet_rays > (1 << 24)) return fail(AA_ERR_INVALID, "packet_rays out of range");
-    for (int32_t i = 0; i < count; ++i) {
-        const aa_batch_trace& t = traces[i];
-        int rc = aa::validate_trace(t.scene, &t.params, t.d_sources, t.d_directions, t.d_diffuse_bank,
-                                    t.d_diffuse_random, t.d_diffuse_indices, &t.outputs);
-        if (rc != AA_OK) return rc;
-        if (t.scene->device != b->device) return fail(AA_ERR_INVALID, "trace %d: scene on another device", i);
-        if (t.params.visual_limit > 0)
-            return fail(AA_ERR_INVALID, "trace %d: visual candidates are recorded by aa_trace only", i);
-    }
+    aa::BatchPlan plan;
+    int rc = aa::plan_batch(b->device, traces, count, tuning, plan);
+    if (rc != AA_OK) return rc;
     std::lock_guard<std::mutex> lock(b->mutex);
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
     AA_HIP(hipSetDevice(b->device));
-    const aa::DeviceProps& props = aa::device_props(b->device);
-    if (!props.valid) return fail(AA_ERR_DEVICE, "cannot query device %d", b->device);
-
-    // LDS plan shared by every trace of the launch: per-wave surface tallies
-    // (histograms go to global atomics)
-    int max_surf = 0;
-    for (int32_t i = 0; i < count; ++i) {
-        const aa_batch_trace& t = traces[i];
-        if (t.params.num_sources == 1) max_surf = std::max(max_surf, t.scene->num_surfaces);
-    }
-    const int waves = wg / 64;
-    const size_t lds_cap = props.lds_per_block > 0 ? props.lds_per_block : 65536;
-    // surface tallies only while the workgroups the register budget allows
-    // (12 waves per CU) stay resident
-    const size_t lds_resident = (props.lds_per_cu > 0 ? props.lds_per_cu : lds_cap) /
-                                (size_t)std::max(1, 12 * 64 / wg);
-    const int surf_cap =
-        (max_surf > 0 && aa::private_bytes(wg) + (size_t)waves * max_surf * 12 <= lds_resident) ? max_surf : 0;
-    const size_t lds = aa::private_bytes(wg) + (size_t)waves * surf_cap * 12;
-    // the kernel's static LDS (the per-wave cursors) comes on top of the
-    // dynamic request
-    const size_t static_lds = (size_t)waves * sizeof(unsigned);
-    if (lds + static_lds > lds_cap) return fail(AA_ERR_LIMIT, "LDS request %zu exceeds %zu bytes", lds, lds_cap);
-
-    int per_cu = tuning->workgroups_per_cu;
-    if (per_cu <= 0) {
-        switch (wg) {
-            case 64: per_cu = occupancy_blocks<64>(lds); break;
-            case 128: per_cu = occupancy_blocks<128>(lds); break;
-            case 256: per_cu = occupancy_blocks<256>(lds); break;
-            default: per_cu = occupancy_blocks<512>(lds); break;
-        }
-    }
-    // packet size: the tuning's, or (0) about two work items per resident
-    // wave so that small batches still spread over the whole GPU
-    int64_t tasks = 0;
-    for (int32_t i = 0; i < count; ++i) {
-        const aa_batch_trace& t = traces[i];
-        const int g = t.params.sources_per_lane > 0 ? std::min(t.params.sources_per_lane, t.params.num_sources)
-                                                    : std::min(8, t.params.num_sources);
-        tasks += (int64_t)t.params.num_rays * ((t.params.num_sources + g - 1) / g);
-    }
-    int packet_rays = tuning->packet_rays;
-    if (packet_rays <= 0) {
-        const int64_t resident = (int64_t)props.cu * per_cu * waves;
-        const int64_t want = (tasks + 2 * resident - 1) / (2 * resident);
-        packet_rays = (int)std::min<int64_t>(16384, std::max<int64_t>(64, (want + 63) / 64 * 64));
-    }
 
     // the pinned table may still be read by the previous submission's copy
     if (b->pending) AA_HIP(hipEventSynchronize(b->copied));
-    const size_t hdr_bytes = (sizeof(aa::BatchHeader) + 255) & ~size_t(255);
-    aa::BatchHeader* h_hdr = static_cast<aa::BatchHeader*>(b->h_table);
-    aa::BatchTrace* h_tr = reinterpret_cast<aa::BatchTrace*>(static_cast<char*>(b->h_table) + hdr_bytes);
-    int64_t packets = 0;
-    for (int32_t i = 0; i < count; ++i) {
-        const aa_batch_trace& t = traces[i];
-        aa::BatchTrace& d = h_tr[i];
-        std::memset(&d, 0, sizeof(d));
-        d.sv = aa::make_scene_view(t.scene, t.params.use_bvh != 0);
-        d.p = aa::make_ray_params(&t.params, t.d_sources, t.d_directions, t.d_diffuse_bank, t.d_diffuse_random,
-                                  t.d_diffuse_indices);
-        d.out = aa::make_outputs(&t.outputs, &t.params);
-        const int64_t trace_tasks = (int64_t)d.p.num_rays * d.p.groups;
-        if (trace_tasks > (int64_t)1 << 30) return fail(AA_ERR_LIMIT, "trace %d has too many lane tasks", i);
-        d.num_packets = (int)((trace_tasks + packet_rays - 1) / packet_rays);
-        d.first_packet = (int)packets;
-        packets += d.num_packets;
-        d.lds_surf = (t.params.num_sources == 1 && t.scene->num_surfaces <= surf_cap) ? 1 : 0;
-        const aa::OutputExtents ext = aa::output_extents(&t.outputs, &t.params, t.scene->num_surfaces);
-        for (int r = 0; r < 8; ++r) {
-            d.zero_ptr[r] = static_cast<unsigned*>(ext.ptr[r]);
-            d.zero_words[r] = (unsigned)(ext.bytes[r] / 4);
-        }
-    }
-    if (packets > (int64_t)1 << 30) return fail(AA_ERR_LIMIT, "too many work items (%lld)", (long long)packets);
-    h_hdr->num_traces = count;
-    h_hdr->total_packets = (int)packets;
-    h_hdr->packet_rays = packet_rays;
-    h_hdr->queue_head = 0u;
-
-    aa::BatchHeader* d_hdr = static_cast<aa::BatchHeader*>(b->d_table);
-    const aa::BatchTrace* d_tr = reinterpret_cast<const aa::BatchTrace*>(static_cast<char*>(b->d_table) + hdr_bytes);
-    const int wave_items = (int)std::min<int64_t>(packets, (int64_t)1 << 30);
-    const int grid = std::max(1, std::min(props.cu * per_cu, (wave_items + waves - 1) / waves));
-    const int persistent_grid = std::max(1, props.cu * per_cu);
+    rc = aa::fill_table(traces, count, plan, b->h_table);
+    if (rc != AA_OK) return rc;
 
     auto enqueue = [&](hipStream_t s) -> hipError_t {
-        hipError_t e = hipMemcpyAsync(b->d_table, b->h_table,
-                                      tuning->use_graph ? b->table_bytes : hdr_bytes + (size_t)count * sizeof(aa::BatchTrace),
-                                      hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(aa::batch_zero_kernel, dim3(tuning->use_graph ? b->max_traces : count), dim3(256), 0, s,
-                           d_hdr, d_tr);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        const int g = tuning->use_graph ? persistent_grid : grid;
-#define AA_LAUNCH(W)                                                             \
-    case W:                                                                      \
-        e = launch_batch<W>(d_hdr, d_tr, g, lds, surf_cap, s);                   \
-        break;
-        switch (wg) {
-            AA_LAUNCH(64)
-            AA_LAUNCH(128)
-            AA_LAUNCH(256)
-            AA_LAUNCH(512)
-        }
-#undef AA_LAUNCH
-        return e;
+        // a graph replays a copy of the whole table and zeroes up to max_traces
+        // entries (the zero kernel skips entries past num_traces)
+        const size_t bytes = tuning->use_graph ? b->table_bytes : aa::table_bytes(count);
+        return aa::enqueue_batch(b->d_table, b->h_table, bytes, tuning->use_graph ? b->max_traces : count, plan,
+                                 tuning->use_graph ? plan.persistent_grid : plan.grid, s);
     };
 
     if (tuning->use_graph) {
-        const auto key = std::make_tuple(wg, 0, 0, surf_cap, persistent_grid, 0, stream);
+        const auto key = std::make_tuple(plan.wg, 0, 0, plan.surf_cap, plan.persistent_grid, 0, stream);
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {
             hipStream_t cap;
