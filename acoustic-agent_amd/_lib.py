@@ -24,6 +24,7 @@ AA_RNG_COUNTER = 1
 AA_NUM_BANDS = 6
 AA_MAX_SOURCES = 64
 AA_CNT_COUNT = 9
+AA_STATIC_BATCH_TRACES = 256
 
 COUNTER_NAMES = (
     "bounce_queries",
@@ -155,6 +156,7 @@ _SIGNATURES = {
     "aa_batch_create": (_I32, [_I32, _I32, ctypes.POINTER(_P)]),
     "aa_batch_submit": (_I32, [_P, ctypes.POINTER(BatchTrace), _I32, ctypes.POINTER(BatchTuning), _P]),
     "aa_batch_release": (_I32, [_P]),
+    "aa_batch_submit_static": (_I32, [_I32, ctypes.POINTER(BatchTrace), _I32, ctypes.POINTER(BatchTuning), _P]),
     "aa_fdn_impulse": (_I32, [_P, _I32, _P, _I32, _I64, _P, _I32, _P, _P]),
 }
 

@@ -304,6 +304,20 @@ class BatchEngine:
                                                ctypes.c_void_p(s.cuda_stream)))
         return self._finish(reqs, lay, arenas, logs, s, copy_back)
 
+    def submit_static(self, requests: Sequence[TraceRequest], tuning: BatchTuning | None = None, stream=None,
+                      order: bool = True) -> BatchResult:
+        """The same batch through ``aa_batch_submit_static``: the library
+        allocates nothing for it (its table is a static device array of
+        ``AA_STATIC_BATCH_TRACES`` entries); the outputs are the arenas
+        allocated here, as the caller's."""
+        tuning = tuning or BatchTuning()
+        table, reqs, lay, arenas, logs = self.pack(requests, tuning, order)
+        s = stream if stream is not None else self.torch.cuda.current_stream(self.device)
+        ts = tuning.as_struct()
+        _lib.check(_lib.load().aa_batch_submit_static(self.device, table, len(reqs), ctypes.byref(ts),
+                                                      ctypes.c_void_p(s.cuda_stream)))
+        return self._finish(reqs, lay, arenas, logs, s, False)
+
 
 def trace_sequential(requests: Sequence[TraceRequest], stream=None,
                      plain: bool = False) -> tuple[list[dict[str, Any]], float]:

@@ -446,3 +446,56 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches without allocation: the descriptor table in a static device array
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// header (rounded to 256 bytes) + AA_STATIC_BATCH_TRACES descriptors
+constexpr size_t kStaticTableBytes =
+    ((sizeof(aa::BatchHeader) + 255) & ~size_t(255)) + (size_t)AA_STATIC_BATCH_TRACES * sizeof(aa::BatchTrace);
+
+// The table of aa_batch_submit_static.  Each GPU holds its own copy (at its
+// own address) for the life of the process.  In host code this name is a
+// host-side stand-in, never a device address: host code uses the address
+// hipGetSymbolAddress returns for the current device.
+__device__ __align__(256) unsigned char scratch[kStaticTableBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_static_mutex;              // guards the pointers and the host staging table
+void* g_static_table[kMaxDevices] = {};  // device address of `scratch` per device
+alignas(256) unsigned char g_static_staging[kStaticTableBytes];  // host table, .bss (no allocation)
+
+}  // namespace
+
+extern "C" int aa_batch_submit_static(int32_t device, const aa_batch_trace* traces, int32_t count,
+                                      const aa_batch_tuning* tuning, void* stream_ptr) {
+    if (!traces || !tuning) return fail(AA_ERR_INVALID, "aa_batch_submit_static: NULL argument");
+    if (device < 0 || device >= kMaxDevices) return fail(AA_ERR_INVALID, "device %d out of range", device);
+    if (count < 1 || count > AA_STATIC_BATCH_TRACES)
+        return fail(AA_ERR_LIMIT, "aa_batch_submit_static takes 1..%d traces (got %d)", AA_STATIC_BATCH_TRACES,
+                    count);
+    if (tuning->use_graph) return fail(AA_ERR_INVALID, "aa_batch_submit_static does not capture graphs");
+    aa::BatchPlan plan;
+    int rc = aa::plan_batch(device, traces, count, tuning, plan);  // selects `device`
+    if (rc != AA_OK) return rc;
+    std::lock_guard<std::mutex> lock(g_static_mutex);
+    if (!g_static_table[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(AA_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+                        hipGetErrorString(e));
+        g_static_table[device] = ptr;
+    }
+    rc = aa::fill_table(traces, count, plan, g_static_staging);
+    if (rc != AA_OK) return rc;
+    // from pageable host memory: the staging table has been read when the
+    // copy call returns, so the next call may refill it
+    const hipError_t e = aa::enqueue_batch(g_static_table[device], g_static_staging, aa::table_bytes(count), count,
+                                           plan, plan.grid, static_cast<hipStream_t>(stream_ptr));
+    if (e != hipSuccess) return fail(AA_ERR_DEVICE, "static batch launch failed: %s", hipGetErrorString(e));
+    return AA_OK;
+}

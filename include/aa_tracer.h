@@ -237,6 +237,23 @@ int aa_batch_submit(aa_batch_t batch, const aa_batch_trace *traces, int32_t coun
                     void *stream);
 int aa_batch_release(aa_batch_t batch);
 
+/* ---- a batch that allocates nothing (no counterpart in the reference) ----
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  Same traces,
+ * tuning and results as aa_batch_submit on `device`, without a batch object:
+ * the descriptor table and the work-queue head live in a static device array
+ * of AA_STATIC_BATCH_TRACES descriptors (about 100 KB, held by each GPU the
+ * library runs on until the process ends), filled from a static host table.
+ * Takes 1..AA_STATIC_BATCH_TRACES traces (more: AA_ERR_LIMIT); use_graph must
+ * be 0.  The call allocates no memory; outputs are the caller's, as for
+ * aa_batch_submit.  Every call on a GPU shares that GPU's table, so calls that
+ * overlap on one GPU (submitted on different streams before the previous one
+ * finished) race: keep one such submission in flight per GPU (calls on one
+ * stream are ordered and safe). */
+#define AA_STATIC_BATCH_TRACES 256
+int aa_batch_submit_static(int32_t device, const aa_batch_trace *traces, int32_t count, const aa_batch_tuning *tuning,
+                           void *stream);
+
 /* ---- host: late-reverberation FDN (replaces _fdn_impulse_kernel,
  *      steam_rt.py:2235-2328) ----
  * Impulse response of num_delays (a power of two) Hadamard-mixed delay lines

@@ -215,3 +215,17 @@ def test_wall_spec_clamps_to_room_height():
     assert room.surfaces[0]["z"] == (0.0, 2.5)
     mesh = tessellate(room.kernel_arrays())
     assert mesh.num_triangles == 2 + 2 + 2
+
+
+def test_static_batch_rejects_before_touching_the_device():
+    lib = _lib.load()
+    tuning = _lib.BatchTuning(256, 0, 0, 0)
+    table = (_lib.BatchTrace * (_lib.AA_STATIC_BATCH_TRACES + 1))()
+    assert lib.aa_batch_submit_static(0, table, _lib.AA_STATIC_BATCH_TRACES + 1, ctypes.byref(tuning), None) == \
+        _lib.AA_ERR_LIMIT
+    assert "1..256" in lib.aa_last_error().decode()
+    assert lib.aa_batch_submit_static(0, table, 0, ctypes.byref(tuning), None) == _lib.AA_ERR_LIMIT
+    assert lib.aa_batch_submit_static(0, None, 1, ctypes.byref(tuning), None) == _lib.AA_ERR_INVALID
+    assert lib.aa_batch_submit_static(-1, table, 1, ctypes.byref(tuning), None) == _lib.AA_ERR_INVALID
+    graph = _lib.BatchTuning(256, 0, 0, 1)
+    assert lib.aa_batch_submit_static(0, table, 1, ctypes.byref(graph), None) == _lib.AA_ERR_INVALID
