@@ -59,6 +59,8 @@ void ao_philox4x32_10(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32
  * The jitter mirrors rng.uniform(-0.02, 0.02) of steam_rt.py:3964 and the
  * Fibonacci direction formula of steam_rt.py:3958-3968.
  */
+#define AO_GOLDEN_QUARTER 0x30E44323405AC1F5ull /* round((3 - sqrt 5) / 4 * 2^64) */
+
 void ao_counter_direction(uint64_t seed, int64_t ray, int64_t num_rays, double out[3])
 {
     uint32_t ctr[4] = {0xFFFFFFFFu, (uint32_t)ray, 0u, 0u};
@@ -74,7 +76,11 @@ void ao_counter_direction(uint64_t seed, int64_t ray, int64_t num_rays, double o
     if (rr < 0.0) rr = 0.0;
     if (rr > 1.0) rr = 1.0;
     double rad = sqrt(rr);
-    double theta = i * (M_PI * (3.0 - sqrt(5.0))) + jitter;
+    /* golden-angle step (i + 1/2) pi (3 - sqrt 5), reduced modulo 2 pi
+       exactly: 2 pi frac((2 ray + 1) K / 2^64) with K = (3 - sqrt 5) / 4
+       in 64-bit fixed point (the product wraps modulo 2^64) */
+    const uint64_t f = (2u * (uint64_t)ray + 1u) * AO_GOLDEN_QUARTER;
+    double theta = 6.283185307179586 * ((double)(f >> 11) * 0x1p-53) + jitter;
     out[0] = cos(theta) * rad;
     out[1] = sin(theta) * rad;
     out[2] = z;

@@ -39,6 +39,26 @@ def test_philox_matches_rocrand_golden_vectors():
         assert np.array_equal(O.philox(ctr, key), out)
 
 
+def test_counter_direction_golden_angle_at_large_ray_indices():
+    # theta = (i + 1/2) pi (3 - sqrt 5) + jitter modulo 2 pi, evaluated in
+    # 50-digit decimal arithmetic; the jitter is U(-0.02, 0.02) from the
+    # Philox word of counter (0xFFFFFFFF, i, 0, 0)
+    from decimal import Decimal, getcontext
+
+    getcontext().prec = 50
+    pi = Decimal("3.14159265358979323846264338327950288419716939937510")
+    golden = pi * (Decimal(3) - Decimal(5).sqrt())
+    n = 200_000
+    for i in (0, 1, 4_097, 123_456, 199_999):
+        w = O.philox((0xFFFFFFFF, i, 0, 0), (1729, 0))
+        jitter = Decimal(-0.02) + Decimal(0.04) * Decimal(int(w[0]) >> 8) / Decimal(16777216)
+        theta = (Decimal(i) + Decimal("0.5")) * golden % (2 * pi) + jitter
+        d = O.counter_direction(1729, i, n)
+        r = math.sqrt(max(0.0, 1.0 - d[2] ** 2))
+        assert d[0] == pytest.approx(math.cos(float(theta)) * r, abs=1e-13)
+        assert d[1] == pytest.approx(math.sin(float(theta)) * r, abs=1e-13)
+
+
 def test_counter_stream_draws():
     # directions: unit vectors on the Fibonacci lattice with |jitter| <= 0.02
     n = 64
