@@ -141,6 +141,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
         Lds lds;
         lds.stack = s_stack;
         lds.queue = s_queue;
+        lds.log = reinterpret_cast<float*>(smem + log_offset(WG));
         lds.hist = nullptr;
         lds.hits = lds_surf ? my_hits : nullptr;
         lds.contrib = lds_surf ? my_contrib : nullptr;

@@ -105,6 +105,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void tr
     Lds lds;
     lds.stack = s_stack;
     lds.queue = s_queue;
+    lds.log = reinterpret_cast<float*>(smem + log_offset(WG));
     lds.hist = LDS_HIST ? s_hist : nullptr;
     lds.hits = a.lds_surf ? s_hits : nullptr;
     lds.contrib = a.lds_surf ? s_contrib : nullptr;
