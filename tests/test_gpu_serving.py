@@ -89,10 +89,10 @@ def test_autotuner_picks_from_space_and_caches(stream, tmp_path):
     reqs = _resolve(traces, cache.acquire(t.scene for t in traces))
     path = tmp_path / "tune.json"
     tuner = Autotuner(0, cache_path=path)
-    space = {"block_size": (128, 256), "packet_rays": (0, 1024, 4096), "workgroups_per_cu": (0,)}
+    space = {"packet_rays": (0, 1024, 4096), "block_size": (256, 128), "workgroups_per_cu": (0,)}
     best = tuner.tune_batch(reqs, space=space, sample=8, reps=1)
     assert best.block_size in space["block_size"] and best.packet_rays in space["packet_rays"]
-    assert len(tuner.log) == 6
+    assert 1 + 2 + 1 <= len(tuner.log) <= 6  # at least one full sweep, never more than the grid
     again = Autotuner(0, cache_path=path).tune_batch(reqs, space=space, sample=8, reps=1)
     assert again.key() == best.key()
     assert set(BATCH_SPACE) >= set(space)
