@@ -269,9 +269,10 @@ def sample_baselines(requests, sample: int, seed: int) -> dict:
             "single_trace_rirs_per_s": round(len(chosen) / single_s, 3)}
 
 
-def c1_single_trace(steps: int, seed: int) -> dict:
+def c1_single_trace(steps: int, seed: int, plain: bool = False) -> dict:
     """The single-scene trace: furnished L plan (~20k triangles), 200k rays,
-    200 bounces, 2.0 s at 1 ms, one persistent launch per trace."""
+    200 bounces, 2.0 s at 1 ms, one persistent launch per trace (``plain``:
+    one launch of the one-thread-per-ray kernel per trace instead)."""
     import torch
 
     w = scenes.l_plan_workload(seed=seed, target_triangles=20_000)
@@ -289,20 +290,22 @@ def c1_single_trace(steps: int, seed: int) -> dict:
         irradiance_min_distance=1.0, specular_exponent=100.0, dipole_weight=0.0, dipole_power=1.0)
     stream = torch.cuda.current_stream(0)
     for _ in range(2):
-        hip_backend.launch_trace(scene, params, src, bank, buffers, stream=stream)
+        hip_backend.launch_trace(scene, params, src, bank, buffers, stream=stream, plain=plain)
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     for a_, b_ in ev:
         a_.record(stream)
-        hip_backend.launch_trace(scene, params, src, bank, buffers, stream=stream)
+        hip_backend.launch_trace(scene, params, src, bank, buffers, stream=stream, plain=plain)
         b_.record(stream)
     torch.cuda.synchronize()
     ms = [a_.elapsed_time(b_) for a_, b_ in ev]
+    counters = dict(zip(_lib.COUNTER_NAMES, (int(x) for x in buffers.counters.cpu().numpy())))
     scene.close()
     return {"workload": f"furnished 3-room L plan, {w.mesh.num_triangles} triangles, 200k rays, 200 bounces, "
-                        "2.0 s @ 1 ms", "kernel_ms_p50": round(float(np.percentile(ms, 50)), 4),
+                        "2.0 s @ 1 ms" + (", plain one-thread-per-ray kernel" if plain else ""),
+            "kernel_ms_p50": round(float(np.percentile(ms, 50)), 4),
             "rirs_per_s": round(1000.0 / float(np.mean(ms)), 2),
-            "primary_rays_per_s": round(N / (float(np.mean(ms)) / 1000.0), 1)}
+            "primary_rays_per_s": round(N / (float(np.mean(ms)) / 1000.0), 1), "counters": counters}
 
 
 def served_report(rates, seconds: float, seed: int, sequential_rates, plain_rates) -> dict:
@@ -492,7 +495,7 @@ def run_keyframes(args) -> int:
 
 
 def run_single(args) -> int:
-    out = c1_single_trace(args.steps, args.seed)
+    out = c1_single_trace(args.steps, args.seed, plain=args.plain)
     print(json.dumps({"metric": "RIRs/s (single-scene trace)", "unit": "RIRs/s", "value": out["rirs_per_s"],
                       "dtype": "f32", "data": "synthetic", "config": out}), flush=True)
     return 0
@@ -523,6 +526,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=8, help="traces in the CPU baseline sample")
     ap.add_argument("--serve-seconds", type=float, default=1.5)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--plain", action="store_true", help="c1 only: the plain one-thread-per-ray kernel on the same trace")
     ap.add_argument("--tuning", type=str, default="", help="JSON BatchTuning fields (skips the autotuner)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the baselines and the served stream")
