@@ -46,6 +46,7 @@
 #define RPOW powf
 #define RFLOOR floorf
 #define RABS fabsf
+#define RFMA fmaf
 #define BVH_EPS 1.0e-5f
 #define PAR_EPS 1.0e-12f
 #else
@@ -56,6 +57,7 @@
 #define RPOW pow
 #define RFLOOR floor
 #define RABS fabs
+#define RFMA fma
 #define BVH_EPS 2.0e-6
 #define PAR_EPS 1.0e-15
 #endif
@@ -180,28 +182,31 @@ static REAL surface_hit(const SCENE *sc, int si, const REAL o[3], const REAL d[3
 }
 
 /* Moller-Trumbore, double-sided, barycentric tolerance 1e-6 (the reference's
- * wall-edge tolerance, steam_rt.py:4380).  The HIP kernel uses the same
- * operation order so that the binary32 build reproduces it exactly. */
+ * wall-edge tolerance, steam_rt.py:4380).  Cross products and dot products
+ * are evaluated with fused multiply-adds: a*b - c*d as fma(a, b, -(c*d)) and
+ * a.b as fma(ax, bx, fma(ay, by, az*bz)).  The HIP kernel uses the same
+ * operations in the same order, so the binary32 build reproduces it
+ * exactly. */
 static REAL tri_hit(const SCENE *sc, int ti, const REAL o[3], const REAL d[3])
 {
     const REAL *v = sc->tri_v + (size_t)ti * 9;
     REAL e1x = v[3] - v[0], e1y = v[4] - v[1], e1z = v[5] - v[2];
     REAL e2x = v[6] - v[0], e2y = v[7] - v[1], e2z = v[8] - v[2];
-    REAL px = d[1] * e2z - d[2] * e2y;
-    REAL py = d[2] * e2x - d[0] * e2z;
-    REAL pz = d[0] * e2y - d[1] * e2x;
-    REAL det = e1x * px + e1y * py + e1z * pz;
+    REAL px = RFMA(d[1], e2z, -(d[2] * e2y));
+    REAL py = RFMA(d[2], e2x, -(d[0] * e2z));
+    REAL pz = RFMA(d[0], e2y, -(d[1] * e2x));
+    REAL det = RFMA(e1x, px, RFMA(e1y, py, e1z * pz));
     if (!(RABS(det) > RS(1e-12))) return BIG;
     REAL inv = RS(1) / det;
     REAL sx = o[0] - v[0], sy = o[1] - v[1], sz = o[2] - v[2];
-    REAL u = (sx * px + sy * py + sz * pz) * inv;
+    REAL u = RFMA(sx, px, RFMA(sy, py, sz * pz)) * inv;
     if (u < RS(-1e-6) || u > RS(1.0 + 1e-6)) return BIG;
-    REAL qx = sy * e1z - sz * e1y;
-    REAL qy = sz * e1x - sx * e1z;
-    REAL qz = sx * e1y - sy * e1x;
-    REAL vv = (d[0] * qx + d[1] * qy + d[2] * qz) * inv;
+    REAL qx = RFMA(sy, e1z, -(sz * e1y));
+    REAL qy = RFMA(sz, e1x, -(sx * e1z));
+    REAL qz = RFMA(sx, e1y, -(sy * e1x));
+    REAL vv = RFMA(d[0], qx, RFMA(d[1], qy, d[2] * qz)) * inv;
     if (vv < RS(-1e-6) || u + vv > RS(1.0 + 1e-6)) return BIG;
-    REAL t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+    REAL t = RFMA(e2x, qx, RFMA(e2y, qy, e2z * qz)) * inv;
     return t > EPS ? t : BIG;
 }
 
