@@ -20,7 +20,8 @@ namespace aa {
 
 constexpr int kBands = AA_NUM_BANDS;
 constexpr int kStack = 40;          // per-lane traversal stack entries (uint16 node indices in LDS)
-constexpr int kMaxWideNodes = 65535; // node indices must fit the 16-bit stack entries
+constexpr int kMaxWideNodes = 65535; // node indices must fit the 16-bit stack entries and sort keys
+constexpr unsigned kNoChild = 0xFFFFFFFFu;  // sort key of a child that is not pushed
 constexpr float kEps = 1e-6f;       // steam_rt_cuda.py:19
 constexpr float kHitOffset = 1e-2f; // steam_rt_cuda.py:20
 constexpr float kBvhEps = 1e-5f;    // steam_rt_cuda.py:21
@@ -551,22 +552,25 @@ __device__ __forceinline__ void traversal_step(const SceneView& sv, unsigned sho
         return;
     }
     const float cull = shadow ? tlim : q.h.t + kBvhEps;
-    float key[4];
-    int idx[4];
+    // Sort key of an internal child: the top 16 bits of its entry distance
+    // (a non-negative float, so its bits order as unsigned integers; 7
+    // mantissa bits) above its 16-bit node index, so that one unsigned
+    // min/max pair swaps both and the stack entry is the key's low half.
+    // Children closer than 1 part in 128 go in index order; the closest
+    // hit does not depend on the order (the tie rule above).  Measured
+    // against a float-key network that swaps the indices alongside:
+    // batched traces +1.0-2.0 %, single trace -1.7 % time.
+    unsigned key[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const bool ok = hit[c] && ch[c] >= 0 && tn[c] <= cull;
-        key[c] = ok ? tn[c] : INFINITY;
-        idx[c] = ch[c];
+        key[c] = ok ? ((__float_as_uint(tn[c]) & 0x7FFF0000u) | (unsigned)ch[c]) : kNoChild;
     }
-#define AA_CSWAP(i, j)           \
-    if (key[j] < key[i]) {       \
-        const float tk = key[i]; \
-        key[i] = key[j];         \
-        key[j] = tk;             \
-        const int ti = idx[i];   \
-        idx[i] = idx[j];         \
-        idx[j] = ti;             \
+#define AA_CSWAP(i, j)                           \
+    {                                            \
+        const unsigned lo = min(key[i], key[j]); \
+        key[j] = max(key[i], key[j]);            \
+        key[i] = lo;                             \
     }
     // nearest first for both kinds: it tightens the culling of bounce
     // queries and makes shadow queries meet their occluder sooner (13 % fewer
@@ -581,13 +585,13 @@ __device__ __forceinline__ void traversal_step(const SceneView& sv, unsigned sho
     for (int c = 3; c >= 1; --c) {
         // the bound is never reached (aa_scene_create caps 3 (depth - 1) at
         // kStack); it only keeps the LDS accesses in range
-        if (key[c] < INFINITY && q.sp < kStack) {
-            stk[q.sp * WG] = (unsigned short)idx[c];
+        if (key[c] != kNoChild && q.sp < kStack) {
+            stk[q.sp * WG] = (unsigned short)key[c];
             ++q.sp;
         }
     }
-    if (key[0] < INFINITY) {
-        q.node = idx[0];
+    if (key[0] != kNoChild) {
+        q.node = (int)(key[0] & 0xFFFFu);
     } else if (q.sp > 0) {
         --q.sp;
         q.node = (int)stk[q.sp * WG];
@@ -637,8 +641,16 @@ __device__ __forceinline__ bool connect(const SceneView& sv, const RayParams& p,
         const float hinv = hn2 > 1e-24f ? __builtin_amdgcn_rsqf(hn2) : 0.0f;
         float cos_half = (hvx * c.nx + hvy * c.ny + hvz * c.nz) * hinv;
         cos_half = cos_half < 0.0f ? 0.0f : cos_half;
-        const float specular =
-            (p.spec_exp + 2.0f) * 0.039788735772973836f * (1.0f - scat) * powf(cos_half, p.spec_exp);
+        // cos_half^n as 2^(n log2 cos_half) with the hardware base-2 logarithm
+        // and exponential (energy only): a relative error of about
+        // n ln2 |log2 cos_half| 2^-22, below 4e-6 wherever the lobe is above
+        // 1e-6 at n = 100; cos_half = 0 gives 2^-inf = 0, and n = 0 gives 1 as
+        // powf does (measured against powf: batched keyframes +3.2 %,
+        // single trace -2.6 % time)
+        const float lobe = p.spec_exp == 0.0f
+                               ? 1.0f
+                               : __builtin_amdgcn_exp2f(p.spec_exp * __builtin_amdgcn_logf(cos_half));
+        const float specular = (p.spec_exp + 2.0f) * 0.039788735772973836f * (1.0f - scat) * lobe;
         const float dm = D > p.dmin ? D : p.dmin;
         const float distance_term = 0.07957747154594767f * __builtin_amdgcn_rcpf(dm * dm);
         // omni source (w = 0): (1 - 0) + 0 * cos = 1 exactly, and 1^p = 1
