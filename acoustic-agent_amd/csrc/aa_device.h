@@ -858,23 +858,16 @@ __device__ __forceinline__ int continue_ray(const SceneView& sv, const RayParams
         const float bx = ny * tz - nz * ty;
         const float by = nz * tx - nx * tz;
         const float bz = nx * ty - ny * tx;
-        float ndx = sx * tx + sy * bx + sz * nx;
-        float ndy = sx * ty + sy * by + sz * ny;
-        float ndz = sx * tz + sy * bz + sz * nz;
-        normalize3(ndx, ndy, ndz);
-        s.dx = ndx;
-        s.dy = ndy;
-        s.dz = ndz;
+        s.dx = sx * tx + sy * bx + sz * nx;
+        s.dy = sx * ty + sy * by + sz * ny;
+        s.dz = sx * tz + sy * bz + sz * nz;
     } else {
         const float dot = s.dx * nx + s.dy * ny + s.dz * nz;
-        float ndx = s.dx - 2.0f * dot * nx;
-        float ndy = s.dy - 2.0f * dot * ny;
-        float ndz = s.dz - 2.0f * dot * nz;
-        normalize3(ndx, ndy, ndz);
-        s.dx = ndx;
-        s.dy = ndy;
-        s.dz = ndz;
+        s.dx = s.dx - 2.0f * dot * nx;
+        s.dy = s.dy - 2.0f * dot * ny;
+        s.dz = s.dz - 2.0f * dot * nz;
     }
+    normalize3(s.dx, s.dy, s.dz);  // both branches end in the same normalisation (one copy in the pass)
     if (s.accum > p.max_path_len) {
         finish_ray(p, out, s, false, bounce + 1, st);
         return kIdle;
