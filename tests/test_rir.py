@@ -133,7 +133,11 @@ def test_reconstructed_rt60_of_an_exponential_decay():
 
 def test_traced_field_to_rir_keeps_the_band_rt60():
     # a traced shoebox field (FP64 oracle) rendered to band signals: the RT60
-    # of the rendered signals follows the echogram's band RT60
+    # of the rendered signals follows the echogram's band RT60 within the
+    # reference's own tolerance for rendered RT60 (12 %,
+    # tests/test_steam_audio_parity.py:516-518; measured here: at most 10.9 %,
+    # in the 250 Hz band, where one second of band-limited noise gives the
+    # Schroeder fit the fewest cycles)
     w = scenes.shoebox_workload(seed=0)
     case = Case(w, num_rays=8192, num_bounces=80, num_bins=1000, seed=3)
     ref = run_oracle(case, "float64")
@@ -146,7 +150,7 @@ def test_traced_field_to_rir_keeps_the_band_rt60():
     rendered = estimate_reconstructed_reverb_times(bands, config)
     for band in FREQUENCY_BANDS:
         assert traced[band] > 0
-        assert rendered[band] == pytest.approx(traced[band], rel=0.15), (band, rendered[band], traced[band])
+        assert rendered[band] == pytest.approx(traced[band], rel=0.12), (band, rendered[band], traced[band])
     # without the late tail the mono RIR is the sum of the reconstructed bands
     dry = SimConfig(fs=16000, seed=3, late_tail=False)
     np.testing.assert_allclose(rir.render_rir(field, dry), np.sum(bands.astype(np.float64), axis=0),
@@ -157,7 +161,7 @@ def test_traced_field_to_rir_keeps_the_band_rt60():
     assert meta["applied"] and meta["transition_start_s"] == pytest.approx(0.225, abs=1 / 16000)
     rendered = estimate_reconstructed_reverb_times(hb, hybrid)
     for band in FREQUENCY_BANDS:
-        assert rendered[band] == pytest.approx(traced[band], rel=0.15), (band, rendered[band], traced[band])
+        assert rendered[band] == pytest.approx(traced[band], rel=0.12), (band, rendered[band], traced[band])
     assert rir.render_rir(field, hybrid).shape == (case.num_bins * 16,)
 
 
