@@ -147,16 +147,28 @@ def estimate_reconstructed_reverb_times(band_rirs: np.ndarray, config: SimConfig
 
 
 def render_band_rirs(field: Mapping[str, Any], config: SimConfig,
-                     rt60_bands: Mapping[str, float] | None = None) -> tuple[np.ndarray, dict[str, Any]]:
+                     rt60_bands: Mapping[str, float] | None = None, *, multi_room: bool = False,
+                     late_reverb_prior: Mapping[str, float] | None = None) -> tuple[np.ndarray, dict[str, Any]]:
     """Band impulse responses [6, n] of a traced field with the FDN late tail
     (``config.late_tail``) at the field's band RT60 (or ``rt60_bands``), and
-    the late-tail metadata."""
+    the late-tail metadata.  For a multi-room scene (steam_rt.py:1015-1035)
+    the tail follows each band's late decay slope where the echogram shows a
+    coupled-space break, and starts its crossfade at that break; a
+    room-system ``late_reverb_prior`` replaces the late targets.  The
+    metadata then carries the profiles as ``late_decay_profiles``."""
     from . import steam_rt as _rt
+    from .decay import apply_coupled_late_reverb_prior, estimate_late_reverb_times
     from .fdn import _render_parametric_fdn_late_reverb
 
     traced = reconstruct_band_irs(field, config)
     rt60 = dict(rt60_bands) if rt60_bands is not None else _rt.estimate_reverb_times(field, config)
-    bands, _, meta = _render_parametric_fdn_late_reverb(traced, field, rt60, config)
+    profiles = None
+    if multi_room:
+        rt60, profiles = estimate_late_reverb_times(field, config, fallback=rt60)
+        rt60, profiles = apply_coupled_late_reverb_prior(rt60, profiles, late_reverb_prior)
+    bands, _, meta = _render_parametric_fdn_late_reverb(traced, field, rt60, config, decay_profiles=profiles)
+    if profiles is not None:
+        meta = {**meta, "late_decay_profiles": profiles}
     return bands, meta
 
 

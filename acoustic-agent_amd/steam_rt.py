@@ -283,3 +283,9 @@ from .rir import (  # noqa: E402,F401
     estimate_reconstructed_reverb_times,
     reconstruct_band_irs,
 )
+# echogram -> decay profiles and late-tail targets, under the reference's names
+from .decay import (  # noqa: E402,F401
+    apply_coupled_late_reverb_prior as _apply_coupled_late_reverb_prior,
+    energy_decay_profile as _energy_decay_profile,
+    estimate_late_reverb_times,
+)
