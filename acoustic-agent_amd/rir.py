@@ -101,31 +101,93 @@ def _air_absorption_amplitude(coefficient_per_m: float, travel_time_s: float, sp
     return math.exp(-max(0.0, float(coefficient_per_m)) * distance_m)
 
 
-def reconstruct_band_irs(field: Mapping[str, Any], config: SimConfig) -> np.ndarray:
-    """Band impulse responses [6, K * ceil(bin * fs)] (float32) of an echogram
-    field (steam_rt.py:2205-2232)."""
+def _bin_ramps(values: np.ndarray, spb: int) -> np.ndarray:
+    """Per-sample values [..., K * spb] rising linearly within each bin from
+    the previous bin's value (the first bin from its own) to the bin's value
+    (steam_rt.py:2222-2228)."""
+    w = np.arange(spb, dtype=np.float64) / spb
+    prev = np.concatenate([values[..., :1], values[..., :-1]], axis=-1)
+    ramp = (1.0 - w) * prev[..., None] + w * values[..., None]
+    return ramp.reshape(*values.shape[:-1], -1)
+
+
+def _band_amplitudes(field: Mapping[str, Any], config: SimConfig) -> tuple[np.ndarray, np.ndarray, int]:
+    """(per-sample amplitude envelopes [6, n] with air absorption, bin
+    energies [6, K] clipped at 0, samples per bin) of an echogram field."""
     from . import steam_rt as _rt  # the absorption table lives there (steam_rt re-exports this module)
 
     echogram = np.asarray(field["echogram"], dtype=np.float64)
     num_bins = int(field["num_bins"])
     fs = int(config.fs)
     spb = max(1, int(math.ceil(float(field["bin_duration_s"]) * fs)))
-    num_samples = num_bins * spb
-    white = np.random.default_rng(config.seed + 7).uniform(-1.0, 1.0, size=num_samples).astype(np.float64)
-    w = np.arange(spb, dtype=np.float64) / spb
     direct_delay_s = max(0.0, float(field.get("direct_delay_s", 0.0)))
     # distance travelled at each bin's centre time (>= 0), for the absorption
     centre_m = (direct_delay_s + (np.arange(num_bins, dtype=np.float64) + 0.5) * spb / fs) * max(0.0, float(config.c))
-    raw = np.zeros((_NUM_BANDS, num_samples), dtype=np.float64)
+    energy = np.clip(echogram[:, :num_bins], 0.0, None)
+    amps = np.sqrt(energy / (4.0 * math.pi))
+    amps[echogram[:, :num_bins] < _ENERGY_THRESHOLD] = 0.0
+    coeffs = np.asarray([max(0.0, float(_rt.AIR_ABSORPTION_NP_PER_M[b])) for b in FREQUENCY_BANDS])
+    air = np.exp(-coeffs[:, None] * centre_m[None, :])  # _air_absorption_amplitude per band and bin
+    return _bin_ramps(amps, spb) * np.repeat(air, spb, axis=1), energy, spb
+
+
+def reconstruct_band_irs(field: Mapping[str, Any], config: SimConfig) -> np.ndarray:
+    """Band impulse responses [6, K * ceil(bin * fs)] (float32) of an echogram
+    field (steam_rt.py:2205-2232)."""
+    env, _, _ = _band_amplitudes(field, config)
+    white = np.random.default_rng(config.seed + 7).uniform(-1.0, 1.0, size=env.shape[1]).astype(np.float64)
+    return bandlimit_band_signals(env * white, int(config.fs))
+
+
+def reconstruct_ambisonic_band_irs(field: Mapping[str, Any], config: SimConfig) -> np.ndarray:
+    """First-order ambisonic band impulse responses [6, 4, n] (float32;
+    channels W, X, Y, Z) of a field with an ``ambisonic_echogram`` [6, 4, K]
+    (steam_rt.py:2680-2724).  Each channel is the band's amplitude envelope
+    times its direction ratio (channel energy over band energy, clipped to
+    [-1, 1]; W = 1; 0 in silent bins), ramped within bins like the
+    amplitudes, on one white-noise sequence shared by all bands and
+    channels.  Without an ambisonic echogram, W carries the band impulse
+    responses and X, Y, Z are silent."""
+    amb = field.get("ambisonic_echogram")
+    if amb is None:
+        mono = reconstruct_band_irs(field, config)
+        out = np.zeros((_NUM_BANDS, 4, mono.shape[1]), dtype=np.float32)
+        out[:, 0] = mono
+        return out
+    env, energy, spb = _band_amplitudes(field, config)
+    amb = np.asarray(amb, dtype=np.float64)[:, :, :energy.shape[1]]
+    live = energy > _ENERGY_THRESHOLD
+    ratios = np.where(live[:, None, :], np.clip(amb / np.maximum(energy, 1e-12)[:, None, :], -1.0, 1.0), 0.0)
+    ratios[:, 0] = live
+    white = np.random.default_rng(config.seed + 11).uniform(-1.0, 1.0, size=env.shape[1]).astype(np.float64)
+    raw = env[:, None, :] * _bin_ramps(ratios, spb) * white
+    out = np.empty(raw.shape, dtype=np.float32)
+    for ch in range(4):
+        out[:, ch] = bandlimit_band_signals(raw[:, ch], int(config.fs))
+    return out
+
+
+def hybridize_ambisonic_tail(traced: np.ndarray, fdn_component: np.ndarray, meta: Mapping[str, Any],
+                             config: SimConfig) -> np.ndarray:
+    """The traced ambisonic bands [6, 4, n] faded out over each band's
+    late-tail transition (gain sqrt of a linear ramp from 1 towards 0, zero
+    after the transition) with the FDN tail [6, n] added to W
+    (steam_rt.py:2654-2677); unchanged when no tail was applied."""
+    out = np.asarray(traced, dtype=np.float64).copy()
+    if not bool(meta.get("applied", False)):
+        return out.astype(np.float32)
+    fs, n = max(1, int(config.fs)), int(out.shape[-1])
+    by_band = meta.get("transition_by_band") if isinstance(meta.get("transition_by_band"), Mapping) else {}
     for bi, band in enumerate(FREQUENCY_BANDS):
-        coeff = _rt.AIR_ABSORPTION_NP_PER_M[band]
-        amps = np.sqrt(np.clip(echogram[bi], 0.0, None) / (4.0 * math.pi))
-        amps[echogram[bi] < _ENERGY_THRESHOLD] = 0.0
-        prev = np.concatenate([amps[:1], amps[:-1]])
-        seg = (1.0 - w)[None, :] * prev[:, None] + w[None, :] * amps[:, None]
-        air = np.exp(-max(0.0, float(coeff)) * centre_m)  # _air_absorption_amplitude per bin
-        raw[bi] = (seg * air[:, None]).reshape(-1) * white
-    return bandlimit_band_signals(raw, fs)
+        tr = by_band.get(band) if isinstance(by_band.get(band), Mapping) else {}
+        start = min(n, max(0, int(math.floor(float(tr.get("start_s", meta["transition_start_s"])) * fs))))
+        end = min(n, max(start + 1, int(math.ceil(float(tr.get("end_s", meta["transition_end_s"])) * fs))))
+        gain = np.ones(n)
+        gain[start:end] = np.sqrt(1.0 - np.arange(end - start) / max(1, end - start))
+        gain[end:] = 0.0
+        out[bi] *= gain
+    out[:, 0] += np.asarray(fdn_component, dtype=np.float64)
+    return out.astype(np.float32)
 
 
 def estimate_reconstructed_reverb_times(band_rirs: np.ndarray, config: SimConfig) -> dict[str, float]:
@@ -146,16 +208,9 @@ def estimate_reconstructed_reverb_times(band_rirs: np.ndarray, config: SimConfig
     return {band: _rt._schroeder_fit_rt60(energy[bi], bin_samples / fs) for bi, band in enumerate(FREQUENCY_BANDS)}
 
 
-def render_band_rirs(field: Mapping[str, Any], config: SimConfig,
-                     rt60_bands: Mapping[str, float] | None = None, *, multi_room: bool = False,
-                     late_reverb_prior: Mapping[str, float] | None = None) -> tuple[np.ndarray, dict[str, Any]]:
-    """Band impulse responses [6, n] of a traced field with the FDN late tail
-    (``config.late_tail``) at the field's band RT60 (or ``rt60_bands``), and
-    the late-tail metadata.  For a multi-room scene (steam_rt.py:1015-1035)
-    the tail follows each band's late decay slope where the echogram shows a
-    coupled-space break, and starts its crossfade at that break; a
-    room-system ``late_reverb_prior`` replaces the late targets.  The
-    metadata then carries the profiles as ``late_decay_profiles``."""
+def _render(field: Mapping[str, Any], config: SimConfig, rt60_bands: Mapping[str, float] | None,
+            multi_room: bool, late_reverb_prior: Mapping[str, float] | None
+            ) -> tuple[np.ndarray, np.ndarray, dict[str, Any]]:
     from . import steam_rt as _rt
     from .decay import apply_coupled_late_reverb_prior, estimate_late_reverb_times
     from .fdn import _render_parametric_fdn_late_reverb
@@ -166,10 +221,36 @@ def render_band_rirs(field: Mapping[str, Any], config: SimConfig,
     if multi_room:
         rt60, profiles = estimate_late_reverb_times(field, config, fallback=rt60)
         rt60, profiles = apply_coupled_late_reverb_prior(rt60, profiles, late_reverb_prior)
-    bands, _, meta = _render_parametric_fdn_late_reverb(traced, field, rt60, config, decay_profiles=profiles)
+    bands, fdn_part, meta = _render_parametric_fdn_late_reverb(traced, field, rt60, config, decay_profiles=profiles)
     if profiles is not None:
         meta = {**meta, "late_decay_profiles": profiles}
+    return bands, fdn_part, meta
+
+
+def render_band_rirs(field: Mapping[str, Any], config: SimConfig,
+                     rt60_bands: Mapping[str, float] | None = None, *, multi_room: bool = False,
+                     late_reverb_prior: Mapping[str, float] | None = None) -> tuple[np.ndarray, dict[str, Any]]:
+    """Band impulse responses [6, n] of a traced field with the FDN late tail
+    (``config.late_tail``) at the field's band RT60 (or ``rt60_bands``), and
+    the late-tail metadata.  For a multi-room scene (steam_rt.py:1015-1035)
+    the tail follows each band's late decay slope where the echogram shows a
+    coupled-space break, and starts its crossfade at that break; a
+    room-system ``late_reverb_prior`` replaces the late targets.  The
+    metadata then carries the profiles as ``late_decay_profiles``."""
+    bands, _, meta = _render(field, config, rt60_bands, multi_room, late_reverb_prior)
     return bands, meta
+
+
+def render_ambisonic_rir(field: Mapping[str, Any], config: SimConfig,
+                         rt60_bands: Mapping[str, float] | None = None, *, multi_room: bool = False,
+                         late_reverb_prior: Mapping[str, float] | None = None) -> np.ndarray:
+    """First-order ambisonic impulse response [4, n] (W, X, Y, Z): the
+    ambisonic band reconstruction with the same late tail as
+    ``render_band_rirs`` hybridised into W, summed over bands
+    (steam_rt.py:1046-1053)."""
+    _, fdn_part, meta = _render(field, config, rt60_bands, multi_room, late_reverb_prior)
+    amb = hybridize_ambisonic_tail(reconstruct_ambisonic_band_irs(field, config), fdn_part, meta, config)
+    return np.sum(amb, axis=0, dtype=np.float64).astype(np.float32)
 
 
 def render_rir(field: Mapping[str, Any], config: SimConfig) -> np.ndarray:

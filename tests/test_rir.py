@@ -192,3 +192,84 @@ def test_fdn_toggle_changes_the_rendered_tail_of_a_traced_room():
     tail_change = float(np.sum(np.square(enabled[start:].astype(np.float64) - disabled[start:])))
     tail_energy = float(np.sum(np.square(disabled[start:].astype(np.float64))))
     assert tail_energy > 0.0 and tail_change > 0.01 * tail_energy, (tail_change, tail_energy)
+
+
+# ---------------------------------------------------------------------------
+# first-order ambisonic reconstruction (steam_rt.py:2654-2724, 1046-1053)
+# ---------------------------------------------------------------------------
+
+
+def _foa_field(energy, ratios, bin_dur=0.0025, delay=0.004):
+    e = np.tile(np.asarray(energy, dtype=np.float64), (6, 1))
+    amb = np.zeros((6, 4, e.shape[1]))
+    for ch in range(4):
+        amb[:, ch] = e * ratios[ch]
+    return {"echogram": e, "ambisonic_echogram": amb, "num_bins": e.shape[1], "bin_duration_s": bin_dur,
+            "direct_delay_s": delay}
+
+
+def test_ambisonic_reconstruction_without_foa_echogram_is_the_mono_one_in_w():
+    field = {"echogram": np.random.default_rng(2).uniform(0, 1, (6, 40)), "num_bins": 40, "bin_duration_s": 0.0025}
+    config = SimConfig(fs=8000, seed=4)
+    amb = rir.reconstruct_ambisonic_band_irs(field, config)
+    assert amb.shape == (6, 4, 40 * 20) and amb.dtype == np.float32
+    np.testing.assert_array_equal(amb[:, 0], reconstruct_band_irs(field, config))
+    assert not np.any(amb[:, 1:])
+
+
+@pytest.mark.parametrize("coeff", [0.0, 0.004])
+def test_ambisonic_channels_are_envelope_times_ratio_on_shared_noise(monkeypatch, coeff):
+    """Equal bands sum back through the crossover: channel c is white noise
+    (seed + 11) x the amplitude envelope x air absorption x the ramped
+    direction ratio, with the ratio clipped to [-1, 1] and 0 in silent bins."""
+    for band in FREQUENCY_BANDS:
+        monkeypatch.setitem(steam_rt.AIR_ABSORPTION_NP_PER_M, band, coeff)
+    config = SimConfig(fs=8000, seed=5)
+    rng = np.random.default_rng(21)
+    e = rng.uniform(0.1, 2.0, 16)
+    e[[4, 9]] = 5e-10  # silent bins
+    ratios = [np.ones(16), rng.uniform(-0.9, 0.9, 16), np.full(16, -2.5), rng.uniform(-0.5, 0.5, 16)]
+    field = _foa_field(e, ratios)
+    spb = 20
+    total = np.sum(rir.reconstruct_ambisonic_band_irs(field, config).astype(np.float64), axis=0)
+    amps = np.sqrt(e / (4 * math.pi))
+    amps[e < 1e-9] = 0.0
+    centre = 0.004 + (np.arange(16) + 0.5) * spb / 8000
+    air = np.repeat(np.exp(-coeff * centre * config.c), spb)
+    white = np.random.default_rng(config.seed + 11).uniform(-1.0, 1.0, 16 * spb)
+    live = e > 1e-9
+    for ch in range(4):
+        r = np.where(live, np.clip(ratios[ch], -1.0, 1.0), 0.0)
+        want = white * _envelope(amps, spb) * air * _envelope(r, spb)
+        np.testing.assert_allclose(total[ch], want, rtol=2e-5, atol=2e-6)
+
+
+def test_ambisonic_tail_fades_the_traced_bands_and_adds_the_fdn_to_w():
+    config = SimConfig(fs=1000)
+    traced = np.ones((6, 4, 100), dtype=np.float32)
+    fdn = np.full((6, 100), 2.0)
+    meta = {"applied": True, "transition_start_s": 0.02, "transition_end_s": 0.06,
+            "transition_by_band": {FREQUENCY_BANDS[1]: {"start_s": 0.03, "end_s": 0.05}}}
+    out = rir.hybridize_ambisonic_tail(traced, fdn, meta, config)
+    for b, (s0, e0) in enumerate([(20, 60), (30, 50)] + [(20, 60)] * 4):
+        gain = np.ones(100)
+        gain[s0:e0] = np.sqrt(1.0 - np.arange(e0 - s0) / (e0 - s0))
+        gain[e0:] = 0.0
+        np.testing.assert_allclose(out[b, 1:], np.tile(gain, (3, 1)), rtol=1e-6)
+        np.testing.assert_allclose(out[b, 0], gain + 2.0, rtol=1e-6)
+    untouched = rir.hybridize_ambisonic_tail(traced, fdn, {"applied": False}, config)
+    np.testing.assert_array_equal(untouched, traced)
+
+
+def test_ambisonic_rir_w_channel_equals_the_mono_rir_without_foa_echogram():
+    # the same traced bands, tail and fade: W of the FOA render is the mono RIR
+    w = scenes.shoebox_workload(seed=0)
+    case = Case(w, num_rays=2048, num_bounces=60, num_bins=800, seed=3)
+    ref = run_oracle(case, "float64")
+    field = {"echogram": ref["echogram"], "num_bins": case.num_bins, "bin_duration_s": case.bin_dur,
+             "direct_delay_s": case.direct_delay(w.source)}
+    config = SimConfig(fs=16000, seed=3, hybrid_transition_s=0.3)
+    foa = rir.render_ambisonic_rir(field, config)
+    assert foa.shape == (4, case.num_bins * 16) and foa.dtype == np.float32
+    np.testing.assert_allclose(foa[0], rir.render_rir(field, config), rtol=1e-5, atol=1e-7)
+    assert not np.any(foa[1:])
