@@ -101,6 +101,22 @@ def _air_absorption_amplitude(coefficient_per_m: float, travel_time_s: float, sp
     return math.exp(-max(0.0, float(coefficient_per_m)) * distance_m)
 
 
+def estimate_signal_reverb_time(signal: np.ndarray, config: SimConfig) -> float:
+    """RT60 of one rendered signal: energy per bin of ``rt_bin_duration_s``,
+    then the Schroeder fit (steam_rt.py:2854-2869)."""
+    from . import steam_rt as _rt
+
+    fs = max(1, int(config.fs))
+    values = np.asarray(signal, dtype=np.float64).reshape(-1)
+    if values.size == 0:
+        return 0.0
+    bin_samples = max(1, int(round(float(config.rt_bin_duration_s) * fs)))
+    num_bins = int(math.ceil(values.size / bin_samples))
+    squared = np.zeros(num_bins * bin_samples, dtype=np.float64)
+    squared[:values.size] = values * values
+    return _rt._schroeder_fit_rt60(squared.reshape(num_bins, bin_samples).sum(axis=1), bin_samples / fs)
+
+
 def _bin_ramps(values: np.ndarray, spb: int) -> np.ndarray:
     """Per-sample values [..., K * spb] rising linearly within each bin from
     the previous bin's value (the first bin from its own) to the bin's value
@@ -208,9 +224,12 @@ def estimate_reconstructed_reverb_times(band_rirs: np.ndarray, config: SimConfig
     return {band: _rt._schroeder_fit_rt60(energy[bi], bin_samples / fs) for bi, band in enumerate(FREQUENCY_BANDS)}
 
 
-def _render(field: Mapping[str, Any], config: SimConfig, rt60_bands: Mapping[str, float] | None,
-            multi_room: bool, late_reverb_prior: Mapping[str, float] | None
-            ) -> tuple[np.ndarray, np.ndarray, dict[str, Any]]:
+def render_with_tail(field: Mapping[str, Any], config: SimConfig, rt60_bands: Mapping[str, float] | None = None,
+                     multi_room: bool = False, late_reverb_prior: Mapping[str, float] | None = None
+                     ) -> tuple[np.ndarray, np.ndarray, dict[str, Any]]:
+    """(band impulse responses [6, n] with the late tail, the tail alone
+    [6, n], metadata): ``render_band_rirs`` with the FDN component kept for
+    the ambisonic hybridisation."""
     from . import steam_rt as _rt
     from .decay import apply_coupled_late_reverb_prior, estimate_late_reverb_times
     from .fdn import _render_parametric_fdn_late_reverb
@@ -237,7 +256,7 @@ def render_band_rirs(field: Mapping[str, Any], config: SimConfig,
     coupled-space break, and starts its crossfade at that break; a
     room-system ``late_reverb_prior`` replaces the late targets.  The
     metadata then carries the profiles as ``late_decay_profiles``."""
-    bands, _, meta = _render(field, config, rt60_bands, multi_room, late_reverb_prior)
+    bands, _, meta = render_with_tail(field, config, rt60_bands, multi_room, late_reverb_prior)
     return bands, meta
 
 
@@ -248,7 +267,7 @@ def render_ambisonic_rir(field: Mapping[str, Any], config: SimConfig,
     ambisonic band reconstruction with the same late tail as
     ``render_band_rirs`` hybridised into W, summed over bands
     (steam_rt.py:1046-1053)."""
-    _, fdn_part, meta = _render(field, config, rt60_bands, multi_room, late_reverb_prior)
+    _, fdn_part, meta = render_with_tail(field, config, rt60_bands, multi_room, late_reverb_prior)
     amb = hybridize_ambisonic_tail(reconstruct_ambisonic_band_irs(field, config), fdn_part, meta, config)
     return np.sum(amb, axis=0, dtype=np.float64).astype(np.float32)
 

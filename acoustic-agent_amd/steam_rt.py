@@ -281,6 +281,7 @@ def estimate_early_decay_times(field: Mapping[str, Any], config: SimConfig) -> d
 from .rir import (  # noqa: E402,F401
     bandlimit_band_signals,
     estimate_reconstructed_reverb_times,
+    estimate_signal_reverb_time,
     hybridize_ambisonic_tail as _hybridize_ambisonic_tail,
     reconstruct_ambisonic_band_irs,
     reconstruct_band_irs,

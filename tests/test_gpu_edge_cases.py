@@ -185,3 +185,34 @@ def test_fdn_toggle_changes_the_rendered_tail_of_a_gpu_traced_room():
     tail_energy = float(np.sum(np.square(disabled[start:])))
     assert tail_energy > 0.0
     assert float(np.sum(np.square(enabled[start:] - disabled[start:]))) > 0.01 * tail_energy
+
+
+def test_reflection_pipeline_on_the_gpu_matches_rendering_its_own_field():
+    """pipeline.simulate_reflections through the HIP trace: a mono receiver
+    with the FOA response and a 4-mic ring; each channel equals the render
+    of its own traced field, and the traced band RT60 is within 2 % of the
+    binary64 oracle on the same counter ray set."""
+    from acoustic_agent_amd import pipeline, rir
+    from acoustic_agent_amd.mic import microphone_array
+    from acoustic_agent_amd.models import SimConfig
+    from acoustic_agent_amd.steam_rt import estimate_reverb_times
+
+    w = scenes.shoebox_workload(seed=0)
+    cfg = SimConfig(fs=8000, duration_s=1.0, rt_duration_s=1.0, rt_bin_duration_s=1e-3, rt_num_rays=8192,
+                    rt_num_bounces=50, hybrid_transition_s=0.3, seed=7)
+    res = pipeline.simulate_reflections(w.scene, w.source, w.listener, config=cfg)
+    assert res.rir.shape == (1, 8000) and res.ambisonic_rir.shape == (4, 8000)
+    fld = res.fields[0]
+    start = int(round(fld["direct_delay_s"] * cfg.fs))
+    bands, _ = rir.render_band_rirs(fld, cfg)
+    want = np.sum(bands, axis=0, dtype=np.float32)[:8000 - start]
+    np.testing.assert_allclose(res.rir[0, start:start + want.size], want, rtol=1e-6, atol=1e-9)
+    case = Case(w, num_rays=8192, num_bounces=50, num_bins=fld["num_bins"], seed=7, rng="counter")
+    ref = run_oracle(case, "float64")
+    ref_rt = estimate_reverb_times({"echogram": ref["echogram"], "bin_duration_s": 1e-3,
+                                    "num_bins": fld["num_bins"]}, cfg)
+    for band, value in res.rt60["traced_rt60_bands"].items():
+        assert value == pytest.approx(ref_rt[band], rel=0.02, abs=1e-3), band
+    arr = pipeline.simulate_reflections(w.scene, w.source, w.listener, config=cfg,
+                                        receiver_model=microphone_array("circular", count=4, radius_m=0.05))
+    assert arr.rir.shape == (4, 8000) and arr.ambisonic_rir is None and len(arr.fields) == 4
