@@ -78,15 +78,50 @@ def _render_channel(fld: Mapping[str, Any], config: SimConfig, multi_room: bool,
     return out
 
 
+def _batched_fields(scene: Any, source: np.ndarray, positions: Sequence[Sequence[float]], config: SimConfig,
+                    emitter: Mapping[str, Any], ambisonics: bool) -> tuple[dict[str, Any], ...]:
+    """The fields of all channels from one persistent batch launch (the same
+    arithmetic and ray set as one ``trace_energy_field`` per channel)."""
+    from .batch import BatchEngine
+    from .hip_backend import cached_scene
+    from .plan import _request
+
+    arrays = scene.kernel_arrays()
+    device_scene = cached_scene(arrays, device=config.device, max_edge=config.rt_tessellation_max_edge_m)
+    reqs = [_request(device_scene, p, source.reshape(1, 3), config, ("channel", i), emitter, ambisonics)
+            for i, p in enumerate(positions)]
+    engine = BatchEngine(config.device, max_traces=len(reqs))
+    try:
+        results = {res["tag"]: res for res in engine.submit(reqs).wait()}
+    finally:
+        engine.close()
+    backend = "bvh" if reqs[0].use_bvh else "linear"
+    fields = []
+    for r in reqs:
+        res = results[r.tag]
+        raw = {k: res[k][0] for k in ("echogram", "hit_counts", "contrib_counts", "surface_energy",
+                                        "actual_bounces", "active_count")}
+        raw.update(ambisonic=res["ambisonic"][0] if ambisonics else None, counters=res["counters"],
+                   device=config.device)
+        direct = float(np.linalg.norm(source - np.asarray(r.listener))) / float(config.c)
+        fields.append(steam_rt.field_from_raw(raw, arrays["names"], r.num_bins, r.bin_dur, direct, ambisonics,
+                                              backend, emitter))
+    return tuple(fields)
+
+
 def simulate_reflections(scene: Any, source: Sequence[float], receiver: Sequence[float], *,
                          config: SimConfig | None = None, receiver_model: Mapping[str, Any] | None = None,
                          source_model: str | Mapping[str, Any] | None = None, multi_room: bool = False,
-                         late_reverb_prior: Mapping[str, float] | None = None) -> ReflectionResult:
+                         late_reverb_prior: Mapping[str, float] | None = None,
+                         batched: bool | None = None) -> ReflectionResult:
     """Reflection impulse response(s) of ``scene`` (a ``RoomScene``) for a
     source and a receiver (engine.py:30-91).  ``multi_room`` applies the
     late-tail targets of coupled spaces (steam_rt.py:1015-1035);
     ``late_reverb_prior`` is the room-system RT60 per band the reference
-    derives from its material model."""
+    derives from its material model.  ``batched`` traces all channels in one
+    persistent launch (default: for several channels with the counter RNG,
+    which the batch kernel draws on the device) instead of one
+    ``trace_energy_field`` per channel."""
     cfg = config or SimConfig()
     model = dict(receiver_model or microphone_array("mono"))
     kind = str(model.get("type", "mono"))
@@ -97,8 +132,14 @@ def simulate_reflections(scene: Any, source: Sequence[float], receiver: Sequence
     array = kind in LAYOUTS
     ambisonics = bool(cfg.render_ambisonics) and not array
     positions = channel_positions(receiver, model)
-    fields = tuple(steam_rt.trace_energy_field(scene, np.asarray(source, dtype=np.float64), np.asarray(p),
-                                               cfg, emitter, render_ambisonics=ambisonics) for p in positions)
+    src = np.asarray(source, dtype=np.float64)
+    if batched is None:
+        batched = len(positions) > 1 and str(cfg.rt_rng).lower() == "counter"
+    if batched:
+        fields = _batched_fields(scene, src, positions, cfg, emitter, ambisonics)
+    else:
+        fields = tuple(steam_rt.trace_energy_field(scene, src, np.asarray(p), cfg, emitter,
+                                                   render_ambisonics=ambisonics) for p in positions)
     channels = [_render_channel(f, cfg, multi_room, late_reverb_prior, ambisonics) for f in fields]
     first = channels[0]
     rt60 = {
@@ -110,7 +151,8 @@ def simulate_reflections(scene: Any, source: Sequence[float], receiver: Sequence
         "edt_bands": _rounded(first["edt"]), "edt_model": "iso3382_0_to_minus10_db_from_path_traced_echogram",
     }
     meta = {"model": "pathtraced_reflections_with_fdn_late_tail", "late_tail": first["meta"],
-            "direct_delay_s": float(fields[0].get("direct_delay_s", 0.0)), "multi_room": bool(multi_room)}
+            "direct_delay_s": float(fields[0].get("direct_delay_s", 0.0)), "multi_room": bool(multi_room),
+            "batched": bool(batched)}
     if multi_room:
         profiles = first["meta"].get("late_decay_profiles", {})
         rt60["late_tail_target_rt60_bands"] = {

@@ -213,6 +213,13 @@ def test_reflection_pipeline_on_the_gpu_matches_rendering_its_own_field():
                                     "num_bins": fld["num_bins"]}, cfg)
     for band, value in res.rt60["traced_rt60_bands"].items():
         assert value == pytest.approx(ref_rt[band], rel=0.02, abs=1e-3), band
-    arr = pipeline.simulate_reflections(w.scene, w.source, w.listener, config=cfg,
-                                        receiver_model=microphone_array("circular", count=4, radius_m=0.05))
-    assert arr.rir.shape == (4, 8000) and arr.ambisonic_rir is None and len(arr.fields) == 4
+    ring = microphone_array("circular", count=4, radius_m=0.05)
+    arr = pipeline.simulate_reflections(w.scene, w.source, w.listener, config=cfg, receiver_model=ring)
+    assert arr.rir.shape == (4, 8000) and arr.ambisonic_rir is None and arr.metadata["batched"]
+    # one persistent batch for the ring == one trace_energy_field per channel
+    one = pipeline.simulate_reflections(w.scene, w.source, w.listener, config=cfg, receiver_model=ring,
+                                        batched=False)
+    for a, b in zip(arr.fields, one.fields):
+        assert a["surface_hit_count"] == b["surface_hit_count"] and a["active_ray_count"] == b["active_ray_count"]
+        assert rel_l1(a["echogram"], b["echogram"]) <= 1e-5
+    np.testing.assert_allclose(arr.rir, one.rir, rtol=1e-3, atol=1e-6 * float(np.abs(one.rir).max()))

@@ -64,8 +64,9 @@ def test_array_receiver_traces_one_channel_per_position_without_ambisonics(trace
     w, calls = traced
     model = microphone_array("circular", count=4, radius_m=0.05)
     config = SimConfig(fs=8000, duration_s=0.8, seed=3)
-    res = pipeline.simulate_reflections(w.scene, w.source, w.listener, config=config, receiver_model=model)
-    assert res.rir.shape == (4, 6400) and res.ambisonic_rir is None
+    res = pipeline.simulate_reflections(w.scene, w.source, w.listener, config=config, receiver_model=model,
+                                        batched=False)
+    assert res.rir.shape == (4, 6400) and res.ambisonic_rir is None and not res.metadata["batched"]
     assert res.metadata["array_channel_count"] == 4
     want = [tuple(np.round(p, 9)) for p in channel_positions(w.listener, model)]
     assert [c["listener"] for c in calls] == want and not any(c["ambisonics"] for c in calls)
