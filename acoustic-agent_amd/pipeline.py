@@ -78,6 +78,35 @@ def _render_channel(fld: Mapping[str, Any], config: SimConfig, multi_room: bool,
     return out
 
 
+def _result(fields: Sequence[Mapping[str, Any]], cfg: SimConfig, model: Mapping[str, Any],
+            emitter: Mapping[str, Any], multi_room: bool, late_reverb_prior: Mapping[str, float] | None,
+            ambisonics: bool, array: bool, batched: bool) -> ReflectionResult:
+    """Render each channel's field and assemble the result (the RT60s are
+    the first channel's, as in engine.py:72-76)."""
+    channels = [_render_channel(f, cfg, multi_room, late_reverb_prior, ambisonics) for f in fields]
+    first = channels[0]
+    rt60 = {
+        "rt60_bands": _rounded(first["rir_bands"]), "rt60_s": round(float(first["rir_s"]), 4),
+        "model": "schroeder_fit_from_final_rir",
+        "traced_rt60_bands": _rounded(first["traced"]), "traced_rt60_s": _mean_positive(first["traced"]),
+        "traced_model": "schroeder_fit_from_path_traced_echogram",
+        "hybrid_rt60_bands": _rounded(first["hybrid"]), "hybrid_rt60_s": _mean_positive(first["hybrid"]),
+        "edt_bands": _rounded(first["edt"]), "edt_model": "iso3382_0_to_minus10_db_from_path_traced_echogram",
+    }
+    meta = {"model": "pathtraced_reflections_with_fdn_late_tail", "late_tail": first["meta"],
+            "direct_delay_s": float(fields[0].get("direct_delay_s", 0.0)), "multi_room": bool(multi_room),
+            "batched": bool(batched)}
+    if multi_room:
+        profiles = first["meta"].get("late_decay_profiles", {})
+        rt60["late_tail_target_rt60_bands"] = {
+            band: float(profiles.get(band, {}).get("selected_target_rt60_s", 0.0)) for band in FREQUENCY_BANDS}
+    if array:
+        meta["array_channel_count"] = len(channels)
+    out = np.stack([c["rir"] for c in channels], axis=0).astype(np.float32)
+    return ReflectionResult(rir=out, rt60=rt60, receiver_model=dict(model), metadata=meta,
+                            ambisonic_rir=first.get("ambisonic"), source_model=dict(emitter), fields=tuple(fields))
+
+
 def _batched_fields(scene: Any, source: np.ndarray, positions: Sequence[Sequence[float]], config: SimConfig,
                     emitter: Mapping[str, Any], ambisonics: bool) -> tuple[dict[str, Any], ...]:
     """The fields of all channels from one persistent batch launch (the same
@@ -109,6 +138,15 @@ def _batched_fields(scene: Any, source: np.ndarray, positions: Sequence[Sequence
     return tuple(fields)
 
 
+def _receiver(receiver_model: Mapping[str, Any] | None, cfg: SimConfig) -> tuple[dict[str, Any], bool, bool]:
+    model = dict(receiver_model or microphone_array("mono"))
+    if str(model.get("type", "mono")) == "hrtf":
+        raise ValueError("binaural (HRTF) rendering is not part of this package; trace a 'mono' receiver with "
+                         "config.render_ambisonics to get the first-order ambisonic response it renders from")
+    array = str(model.get("type", "mono")) in LAYOUTS
+    return model, array, bool(cfg.render_ambisonics) and not array
+
+
 def simulate_reflections(scene: Any, source: Sequence[float], receiver: Sequence[float], *,
                          config: SimConfig | None = None, receiver_model: Mapping[str, Any] | None = None,
                          source_model: str | Mapping[str, Any] | None = None, multi_room: bool = False,
@@ -123,14 +161,8 @@ def simulate_reflections(scene: Any, source: Sequence[float], receiver: Sequence
     which the batch kernel draws on the device) instead of one
     ``trace_energy_field`` per channel."""
     cfg = config or SimConfig()
-    model = dict(receiver_model or microphone_array("mono"))
-    kind = str(model.get("type", "mono"))
-    if kind == "hrtf":
-        raise ValueError("binaural (HRTF) rendering is not part of this package; trace a 'mono' receiver with "
-                         "config.render_ambisonics to get the first-order ambisonic response it renders from")
+    model, array, ambisonics = _receiver(receiver_model, cfg)
     emitter = source_directivity(source_model)
-    array = kind in LAYOUTS
-    ambisonics = bool(cfg.render_ambisonics) and not array
     positions = channel_positions(receiver, model)
     src = np.asarray(source, dtype=np.float64)
     if batched is None:
@@ -140,25 +172,47 @@ def simulate_reflections(scene: Any, source: Sequence[float], receiver: Sequence
     else:
         fields = tuple(steam_rt.trace_energy_field(scene, src, np.asarray(p), cfg, emitter,
                                                    render_ambisonics=ambisonics) for p in positions)
-    channels = [_render_channel(f, cfg, multi_room, late_reverb_prior, ambisonics) for f in fields]
-    first = channels[0]
-    rt60 = {
-        "rt60_bands": _rounded(first["rir_bands"]), "rt60_s": round(float(first["rir_s"]), 4),
-        "model": "schroeder_fit_from_final_rir",
-        "traced_rt60_bands": _rounded(first["traced"]), "traced_rt60_s": _mean_positive(first["traced"]),
-        "traced_model": "schroeder_fit_from_path_traced_echogram",
-        "hybrid_rt60_bands": _rounded(first["hybrid"]), "hybrid_rt60_s": _mean_positive(first["hybrid"]),
-        "edt_bands": _rounded(first["edt"]), "edt_model": "iso3382_0_to_minus10_db_from_path_traced_echogram",
-    }
-    meta = {"model": "pathtraced_reflections_with_fdn_late_tail", "late_tail": first["meta"],
-            "direct_delay_s": float(fields[0].get("direct_delay_s", 0.0)), "multi_room": bool(multi_room),
-            "batched": bool(batched)}
-    if multi_room:
-        profiles = first["meta"].get("late_decay_profiles", {})
-        rt60["late_tail_target_rt60_bands"] = {
-            band: float(profiles.get(band, {}).get("selected_target_rt60_s", 0.0)) for band in FREQUENCY_BANDS}
-    if array:
-        meta["array_channel_count"] = len(channels)
-    out = np.stack([c["rir"] for c in channels], axis=0).astype(np.float32)
-    return ReflectionResult(rir=out, rt60=rt60, receiver_model=model, metadata=meta,
-                            ambisonic_rir=first.get("ambisonic"), source_model=emitter, fields=fields)
+    return _result(fields, cfg, model, emitter, multi_room, late_reverb_prior, ambisonics, array, batched)
+
+
+def simulate_motion(scene: Any, frames: Sequence[Mapping[str, Any]], *, config: SimConfig | None = None,
+                    receiver_model: Mapping[str, Any] | None = None,
+                    source_model: str | Mapping[str, Any] | None = None, multi_room: bool = False,
+                    late_reverb_prior: Mapping[str, float] | None = None) -> tuple[ReflectionResult, ...]:
+    """One ``ReflectionResult`` per motion frame ``{"source", "receiver"}``
+    (the reference's frame loop, api.py:669-713), traced in one persistent
+    batch: the frames that share a receiver position are one multi-source
+    trace per channel (ray trajectories do not depend on the source), so a
+    frame's result equals its own ``simulate_reflections``."""
+    from .batch import BatchEngine
+    from .hip_backend import cached_scene
+    from .plan import motion_traces
+
+    cfg = config or SimConfig()
+    model, array, ambisonics = _receiver(receiver_model, cfg)
+    emitter = source_directivity(source_model)
+    arrays = scene.kernel_arrays()
+    device_scene = cached_scene(arrays, device=cfg.device, max_edge=cfg.rt_tessellation_max_edge_m)
+    mplan = motion_traces(device_scene, frames, cfg, model, emitter, ambisonics)
+    engine = BatchEngine(cfg.device, max_traces=len(mplan.traces))
+    try:
+        by_tag = {res["tag"]: res for res in engine.submit(mplan.traces).wait()}
+    finally:
+        engine.close()
+    backend = "bvh" if mplan.traces[0].use_bvh else "linear"
+    out = []
+    for f, pairs in enumerate(mplan.frames):
+        src = np.asarray(frames[f]["source"], dtype=np.float64)[:3]
+        fields = []
+        for t, k in pairs:
+            req = mplan.traces[t]
+            res = by_tag[req.tag]
+            raw = {key: res[key][k] for key in ("echogram", "hit_counts", "contrib_counts", "surface_energy",
+                                                "actual_bounces", "active_count")}
+            raw.update(ambisonic=res["ambisonic"][k] if ambisonics else None, counters=res["counters"],
+                       device=cfg.device)
+            direct = float(np.linalg.norm(src - np.asarray(req.listener))) / float(cfg.c)
+            fields.append(steam_rt.field_from_raw(raw, arrays["names"], req.num_bins, req.bin_dur, direct,
+                                                  ambisonics, backend, emitter))
+        out.append(_result(fields, cfg, model, emitter, multi_room, late_reverb_prior, ambisonics, array, True))
+    return tuple(out)

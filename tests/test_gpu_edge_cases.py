@@ -223,3 +223,31 @@ def test_reflection_pipeline_on_the_gpu_matches_rendering_its_own_field():
         assert a["surface_hit_count"] == b["surface_hit_count"] and a["active_ray_count"] == b["active_ray_count"]
         assert rel_l1(a["echogram"], b["echogram"]) <= 1e-5
     np.testing.assert_allclose(arr.rir, one.rir, rtol=1e-3, atol=1e-6 * float(np.abs(one.rir).max()))
+
+
+def test_motion_frames_in_one_batch_equal_each_frame_alone():
+    """pipeline.simulate_motion: 5 frames over 2 receiver positions become
+    two multi-source traces; every frame's field and RIR equal its own
+    simulate_reflections (one trace_energy_field)."""
+    from acoustic_agent_amd import pipeline
+    from acoustic_agent_amd.models import SimConfig
+
+    w = scenes.shoebox_workload(seed=0)
+    cfg = SimConfig(fs=8000, duration_s=0.6, rt_duration_s=0.6, rt_bin_duration_s=1e-3, rt_num_rays=4096,
+                    rt_num_bounces=40, hybrid_transition_s=0.2, seed=5)
+    rng = np.random.default_rng(9)
+    receivers = [w.listener, w.listener + np.asarray((-0.8, 0.5, 0.1))]
+    frames = [{"source": w.source + rng.uniform(-0.3, 0.3, 3), "receiver": receivers[f % 2]} for f in range(5)]
+    moving = pipeline.simulate_motion(w.scene, frames, config=cfg, source_model="cardioid")
+    assert len(moving) == 5
+    for frame, res in zip(frames, moving):
+        alone = pipeline.simulate_reflections(w.scene, frame["source"], frame["receiver"], config=cfg,
+                                              source_model="cardioid")
+        a, b = res.fields[0], alone.fields[0]
+        assert a["surface_hit_count"] == b["surface_hit_count"]
+        assert a["surface_contribution_count"] == b["surface_contribution_count"]
+        assert a["direct_delay_s"] == pytest.approx(b["direct_delay_s"], rel=1e-12)
+        assert rel_l1(a["echogram"], b["echogram"]) <= 1e-5
+        assert rel_l1(a["ambisonic_echogram"], b["ambisonic_echogram"]) <= 1e-5
+        np.testing.assert_allclose(res.rir, alone.rir, rtol=1e-3, atol=1e-6 * float(np.abs(alone.rir).max()))
+        assert res.rt60["traced_rt60_bands"] == pytest.approx(alone.rt60["traced_rt60_bands"], rel=1e-3)

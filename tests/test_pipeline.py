@@ -99,3 +99,10 @@ def test_hrtf_receivers_are_refused(traced):
     with pytest.raises(ValueError, match="HRTF"):
         pipeline.simulate_reflections(w.scene, w.source, w.listener, receiver_model={"type": "hrtf"})
     assert not calls
+
+
+def test_motion_refuses_hrtf_and_empty_frames():
+    w = scenes.shoebox_workload(seed=0)
+    with pytest.raises(ValueError, match="HRTF"):
+        pipeline.simulate_motion(w.scene, [{"source": w.source, "receiver": w.listener}],
+                                 receiver_model={"type": "hrtf"})
