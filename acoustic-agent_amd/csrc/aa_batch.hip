@@ -147,7 +147,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
         lds.contrib = lds_surf ? my_contrib : nullptr;
         lds.senergy = lds_surf ? my_energy : nullptr;
         LaneStats st;
-        trace_rays_refill<WG, false, false>(sv, p, out, lds, &s_cursor[wave], begin, count, st);
+        trace_rays_refill<WG, false, false, true>(sv, p, out, lds, &s_cursor[wave], begin, count, st);
         flush_stats(st, p, out);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -310,6 +310,10 @@ int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& pla
         d.sv = make_scene_view(t.scene, t.params.use_bvh != 0);
         d.p = make_ray_params(&t.params, t.d_sources, t.d_directions, t.d_diffuse_bank, t.d_diffuse_random,
                               t.d_diffuse_indices);
+        // interleaved ray order for multi-source traces only (measured:
+        // 64-keyframe traces +2.9 %; single-source mic arrays -0.3 %, whose
+        // packets lose the coherence of neighbouring first-bounce rays)
+        if (d.p.num_sources <= 1) d.p.ray_stride = 1;
         d.out = make_outputs(&t.outputs, &t.params);
         const int64_t trace_tasks = (int64_t)d.p.num_rays * d.p.groups;
         if (trace_tasks > (int64_t)1 << 30) return fail(AA_ERR_LIMIT, "trace %d has too many lane tasks", i);

@@ -13,6 +13,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <numeric>
+
 #include "../../include/aa_tracer.h"
 #include "aa_internal.h"
 
@@ -73,6 +76,7 @@ struct RayParams {
     int num_rays, num_bounces, num_bins, num_samples, num_sources, rng_mode, amb;
     int src_group;  // sources handled by one lane (one task = one ray x one group)
     int groups;     // ceil(num_sources / src_group); tasks = num_rays * groups, group-major
+    int ray_stride; // INTERLEAVE order: task t of a group traces ray (t * ray_stride) mod num_rays
     unsigned seed_lo, seed_hi;
     float lx, ly, lz, fx, fy, fz;
     float bin_dur, c, max_path_len, listener_radius, source_radius, dmin, spec_exp, dip_w, dip_p;
@@ -383,12 +387,21 @@ __device__ __forceinline__ int visual_slot(const Outputs& out, int ray) {
 
 // Start lane task `task` (ray x source group); returns false if the ray has
 // no bounce to trace (num_bounces == 0).  VISUAL: the caller may record
-// visual candidates (the batch kernel never does).
-template <bool VISUAL = true>
+// visual candidates (the batch kernel never does).  INTERLEAVE: consecutive
+// tasks take rays a golden-ratio stride apart.  Consecutive rays of the
+// lattice share an elevation, and the steep ones all run to the bounce
+// limit (short segments between floor and ceiling), so in ray order they
+// fill whole waves whose lanes then walk in lockstep to the end; spread
+// over the waves, the longest rays finish sooner (measured on a 200k-ray
+// single trace: 7.60 -> 7.39 ms).  Every ray is traced with the same
+// arithmetic in either order.
+template <bool VISUAL = true, bool INTERLEAVE = false>
 __device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out, int task, RayState& s,
                                           LaneStats& st) {
     const int group = task / p.num_rays;  // group-major: a wave's claims mostly share a group
-    const int ray = task - group * p.num_rays;
+    const int local = task - group * p.num_rays;
+    const int ray = INTERLEAVE ? (int)(((unsigned long long)local * (unsigned)p.ray_stride) % (unsigned)p.num_rays)
+                               : local;
     s.ray = ray;
     s.group = group;
     s.group_mask = group_source_mask(p, group);
@@ -1081,7 +1094,7 @@ constexpr int kStepsPerRound = 8;
 // `first`) with the calling wave's lanes.  A lane holds a ray (bounce queries,
 // and its own connections when the queue is full) or a parked connection of
 // another lane's ray; idle lanes take parked connections first, then new rays.
-template <int WG, bool LDS_HIST, bool VISUAL = true>
+template <int WG, bool LDS_HIST, bool VISUAL = true, bool INTERLEAVE = false>
 __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const Outputs& out, const Lds& lds,
                                   unsigned* cursor, int first, int count, LaneStats& st) {
     unsigned short* stk = lds.stack + threadIdx.x;
@@ -1185,7 +1198,7 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
             if (!exhausted) {
                 const bool need = phase == kIdle;
                 const int got = claim_rays(need, cursor, count);
-                if (need && got >= 0 && start_ray<VISUAL>(p, out, first + got, s, st)) {
+                if (need && got >= 0 && start_ray<VISUAL, INTERLEAVE>(p, out, first + got, s, st)) {
                     begin_query(q, s.ox, s.oy, s.oz, s.dx, s.dy, s.dz, kBig);
                     ++st.q_b;
                     phase = kBounce;
@@ -1268,6 +1281,15 @@ inline SceneView make_scene_view(const aa_scene_s* s, bool use_bvh) {
     return sv;
 }
 
+// The integer nearest to n (sqrt 5 - 1) / 2 that is coprime to n (1 for n < 3),
+// so that t -> (t * stride) mod n visits every ray once.
+inline int golden_stride(int n) {
+    if (n < 3) return 1;
+    for (long long s = std::llround(0.6180339887498949 * n); s < n; ++s)
+        if (std::gcd(s, (long long)n) == 1) return (int)s;
+    return 1;
+}
+
 inline RayParams make_ray_params(const aa_trace_params* p, const float* sources, const float* directions,
                                  const float* bank, const float* urand, const int* uidx) {
     RayParams r;
@@ -1284,6 +1306,7 @@ inline RayParams make_ray_params(const aa_trace_params* p, const float* sources,
     r.src_group = p->sources_per_lane > 0 ? p->sources_per_lane : 8;
     if (r.src_group > r.num_sources) r.src_group = r.num_sources;
     r.groups = (r.num_sources + r.src_group - 1) / r.src_group;
+    r.ray_stride = golden_stride(r.num_rays);
     r.rng_mode = p->rng_mode;
     r.amb = p->render_ambisonics ? 1 : 0;
     r.seed_lo = (unsigned)(p->seed & 0xffffffffull);

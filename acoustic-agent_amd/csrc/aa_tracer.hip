@@ -114,7 +114,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void tr
     // persistent lanes: each wave claims rays from the trace's cursor as its
     // lanes' rays end
     unsigned* cursor = reinterpret_cast<unsigned*>(a.out.counters + AA_CNT_RAYS_CLAIMED);
-    trace_rays_refill<WG, LDS_HIST>(a.sv, a.p, a.out, lds, cursor, 0, a.p.num_rays * a.p.groups, st);
+    trace_rays_refill<WG, LDS_HIST, true, true>(a.sv, a.p, a.out, lds, cursor, 0, a.p.num_rays * a.p.groups, st);
     flush_stats(st, a.p, a.out);
     __syncthreads();
     if (LDS_HIST) {
