@@ -467,9 +467,10 @@ __device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out
 //
 // A lane is IDLE (needs a ray), in a BOUNCE query (closest hit of the ray's
 // current segment) or in a SHADOW query (any hit towards one source).  Every
-// round advances each busy lane's query by up to kStepsPerRound BVH nodes
-// (each with its leaf triangles), so the lanes of a wave run the same
-// traversal code whatever bounce or query they are in.  When a query ends, the lane does the
+// round advances each busy lane's query by up to kStepsPerRound steps (a
+// node visit, or the wave's leaf pass for the lanes waiting on hit leaves),
+// so the lanes of a wave run the same traversal code whatever bounce or
+// query they are in.  When a query ends, the lane does the
 // reference's work for that point of the bounce loop (steam_rt.py:4743-4848)
 // and sets up its next query:
 //   bounce done -> termination tests, hit counts, then one connection per
@@ -489,7 +490,8 @@ enum : int { kIdle = 0, kBounce = 1, kShadow = 2 };
 struct Query {
     Ray r;
     float tmax;   // shadow: distance to the source
-    int node;     // next node to visit; -1 when the query is finished
+    int node;     // next node to visit; -1 when the walk is over
+    int pend;     // leaves waiting for the wave's leaf pass: visited node | hit leaf slots << 16, or -1
     int sp;       // stack depth
     Closest h;    // bounce: best hit so far
     bool occluded;
@@ -517,6 +519,7 @@ __device__ __forceinline__ void begin_query(Query& q, float ox, float oy, float 
     q.r.iz = safe_inv(dz);
     q.tmax = tmax;
     q.node = 0;
+    q.pend = -1;
     q.sp = 0;
     q.h.t = kBig;
     q.h.surf = -1;
@@ -567,13 +570,18 @@ __device__ __forceinline__ void test_leaf(const SceneView& sv, int start, int co
     }
 }
 
-// One traversal step of `q`: visit one 4-wide node and the triangles of its
-// hit leaf children, then push the hit internal children nearest first and
-// move to the nearest (or pop).  Bounce queries cull boxes beyond the best
-// hit; shadow queries beyond the source and stop at the first occluder.
+// One node visit of `q`: the slab test of its 4 children, then the hit
+// internal children are pushed nearest first (culled against the best hit
+// so far: bounce queries beyond it, shadow queries beyond the source) and
+// the nearest is visited next (or the stack is popped).  Hit leaf children
+// are not tested here: they are recorded in q.pend and the lane waits for
+// the wave's next leaf pass (leaf_pass), which tests them before the lane
+// visits another node, so a query makes the same decisions as when it tests
+// its leaves at once, except that children are culled against the best hit
+// from before this node's leaves (a few node visits in 10^3 more).
 template <int WG>
-__device__ __forceinline__ void traversal_step(const SceneView& sv, unsigned short* stk,
-                                               Query& q, bool shadow, unsigned& node_cnt, unsigned& tri_cnt) {
+__device__ __forceinline__ void node_step(const SceneView& sv, unsigned short* stk, Query& q, bool shadow,
+                                          unsigned& node_cnt) {
     ++node_cnt;
     float4 v[7];
     load_node(sv, q.node, q.r, v);
@@ -585,19 +593,7 @@ __device__ __forceinline__ void traversal_step(const SceneView& sv, unsigned sho
     unsigned leaves = 0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) leaves |= (hit[c] && ch[c] <= -2) ? (1u << c) : 0u;
-    while (leaves && !q.occluded) {
-        const int c = __builtin_ctz(leaves);
-        leaves &= leaves - 1u;
-        // branch-free pick of child c
-        const int lo = (c & 1) ? ch[1] : ch[0], hi = (c & 1) ? ch[3] : ch[2];
-        const int code = -(((c & 2) ? hi : lo) + 2);
-        test_leaf(sv, code >> 3, (code & 7) + 1, q, shadow, tri_cnt);
-    }
-    if (q.occluded) {
-        q.node = -1;
-        return;
-    }
-    const float cull = shadow ? tlim : q.h.t + kBvhEps;
+    if (leaves) q.pend = q.node | (int)(leaves << 16);
     // Sort key of an internal child: the top 16 bits of its entry distance
     // (a non-negative float, so its bits order as unsigned integers; 7
     // mantissa bits) above its 16-bit node index, so that one unsigned
@@ -609,7 +605,7 @@ __device__ __forceinline__ void traversal_step(const SceneView& sv, unsigned sho
     unsigned key[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const bool ok = hit[c] && ch[c] >= 0 && tn[c] <= cull;
+        const bool ok = hit[c] && ch[c] >= 0;
         key[c] = ok ? ((__float_as_uint(tn[c]) & 0x7FFF0000u) | (unsigned)ch[c]) : kNoChild;
     }
 #define AA_CSWAP(i, j)                           \
@@ -644,6 +640,27 @@ __device__ __forceinline__ void traversal_step(const SceneView& sv, unsigned sho
     } else {
         q.node = -1;
     }
+}
+
+// The leaf pass of a lane with pending leaves: the triangles of the hit leaf
+// children of node q.pend & 0xFFFF (slots in the bits above), in slot order;
+// a shadow query ends at its first occluder.  The child row is read again
+// (its line was just loaded by the node visit).
+__device__ __forceinline__ void leaf_pass(const SceneView& sv, Query& q, bool shadow, unsigned& tri_cnt) {
+    const int node = q.pend & 0xFFFF;
+    unsigned leaves = (unsigned)q.pend >> 16;
+    q.pend = -1;
+    const global_cfloat* g = (const global_cfloat*)(reinterpret_cast<const char*>(sv.nodes) + ((size_t)node << 7) + 96);
+    const int ch[4] = {__float_as_int(g[0]), __float_as_int(g[1]), __float_as_int(g[2]), __float_as_int(g[3])};
+    while (leaves && !q.occluded) {
+        const int c = __builtin_ctz(leaves);
+        leaves &= leaves - 1u;
+        // branch-free pick of child c
+        const int lo = (c & 1) ? ch[1] : ch[0], hi = (c & 1) ? ch[3] : ch[2];
+        const int code = -(((c & 2) ? hi : lo) + 2);
+        test_leaf(sv, code >> 3, (code & 7) + 1, q, shadow, tri_cnt);
+    }
+    if (q.occluded) q.node = -1;
 }
 
 // A shadow connection from the current hit to one source, with its deposit
@@ -1116,12 +1133,24 @@ __device__ __forceinline__ int claim_rays(bool need, unsigned* cursor, int limit
 // wave).
 constexpr int kFinishEighths = 7;
 
-// Node visits per loop round of a traversing lane.  Measured on MI355X
-// against one visit per round: 2 / 3 / 4 / 8 / 16 / 32 visits took a
-// 200k-ray single trace -2 / -5 / -7 / -7 / -7 / -8 % time and batched traces
-// +10..14 % throughput (4..16 within noise of each other).  Ending the round
-// as soon as the completion threshold is met was no better.
+// Steps per loop round of a traversing lane (node visits and leaf passes).
+// Measured on MI355X against one visit per round, with leaves tested at
+// once: 2 / 3 / 4 / 8 / 16 / 32 visits took a 200k-ray single trace -2 / -5 /
+// -7 / -7 / -7 / -8 % time and batched traces +10..14 % throughput (4..16
+// within noise of each other).  Ending the round as soon as the completion
+// threshold is met was no better.  With gated leaf passes, 16 steps instead
+// of 8: batched mic arrays -3.7 %, batched keyframes -1.6 %.
 constexpr int kStepsPerRound = 8;
+
+// Leaf pass gate: the waiting lanes' leaves are tested once
+// waiting * kLeafGateDen >= walking * kLeafGateNum (or no lane walks).
+// Measured on MI355X (waiting : walking = 1:2 / 1:1 / 2:1 / 4:1 / 8:1 /
+// only when none walks): batched mic arrays 746 / 758 / 772 / 768 / 751 /
+// 679 RIRs/s, batched keyframes 1454 / 1457 / 1472 / 1440 / 1414 / 1327,
+// 200k-ray single trace 6.24 / 6.25 / 6.25 / 6.30 / 6.46 / 6.77 ms; leaves
+// tested at once: 681 / 1344 / 6.78 ms.
+constexpr int kLeafGateNum = 2;
+constexpr int kLeafGateDen = 1;
 
 // Trace the tasks [first, first + count) handed out by `cursor` (relative to
 // `first`) with the calling wave's lanes.  A lane holds a ray (bounce queries,
@@ -1150,7 +1179,7 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
     int q_head = 0, q_count = 0;  // wave-uniform ring state of the parked connections
     bool exhausted = false;  // wave-uniform: a claim came back empty, so the pool is drained
     while (true) {
-        bool walking = phase != kIdle && q.node >= 0;
+        bool walking = phase != kIdle && (q.node >= 0 || q.pend >= 0);
         const int n_act = __popcll(__ballot(phase != kIdle));
         const int n_trav = __popcll(__ballot(walking));
         const bool drained = exhausted && q_count == 0;  // nothing left for idle lanes
@@ -1238,7 +1267,7 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
                 }
                 exhausted = __any(need && got < 0) != 0;
             }
-            walking = phase != kIdle && q.node >= 0;
+            walking = phase != kIdle && (q.node >= 0 || q.pend >= 0);
         }
         if (walking) {
             const bool shadow = phase == kShadow;
@@ -1252,9 +1281,23 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
                 test_leaf(sv, 0, sv.num_tris, q, shadow, tris);
                 q.node = -1;
             } else {
+                // Leaf tests are gathered: a lane whose node visit hits leaf
+                // children waits (q.pend) while the others walk on, and the
+                // wave tests the waiting lanes' leaves together once the gate
+                // opens (kLeafGateNum : kLeafGateDen waiting : walking lanes,
+                // or none walks).  A leaf's triangle pair costs about a node
+                // visit, and tested at once (about one pass per step) it ran
+                // with a few lanes of the wave.
 #pragma unroll 1
-                for (int k = 0; k < kStepsPerRound && q.node >= 0; ++k)
-                    traversal_step<WG>(sv, stk, q, shadow, nodes, tris);
+                for (int k = 0; k < kStepsPerRound; ++k) {
+                    const int n_step = __popcll(__ballot(q.node >= 0 && q.pend < 0));
+                    const int n_pend = __popcll(__ballot(q.pend >= 0));
+                    if (n_pend > 0 && n_pend * kLeafGateDen >= n_step * kLeafGateNum && q.pend >= 0)
+                        leaf_pass(sv, q, shadow, tris);
+                    const bool step = q.node >= 0 && q.pend < 0;
+                    if (!__any(step)) break;  // only when no lane waits either
+                    if (step) node_step<WG>(sv, stk, q, shadow, nodes);
+                }
             }
             if (shadow) {
                 st.ns += nodes;
