@@ -106,6 +106,7 @@ class TraceOutputs(ctypes.Structure):
         ("d_visual_orders", ctypes.c_void_p),
         ("d_visual_distances", ctypes.c_void_p),
         ("d_visual_gains", ctypes.c_void_p),
+        ("d_ray_times", ctypes.c_void_p),
     ]
 
 

@@ -104,6 +104,7 @@ struct Outputs {
     float* vis_dist;    // [V]
     float* vis_gain;    // [V]
     int vis_limit, vis_stride, vis_bounces;
+    unsigned long long* ray_times;  // [rays, 2] start / end ticks of source group 0, or null
 };
 
 // Per-lane tallies flushed once per wave.
@@ -388,10 +389,17 @@ __device__ __forceinline__ unsigned long long full_source_mask(int Ks) {
     return Ks >= 64 ? ~0ull : ((1ull << Ks) - 1ull);
 }
 
+// Start / end stamps of a ray (source group 0) on the constant-rate
+// realtime clock, when the caller asked for them.
+__device__ __forceinline__ void stamp_ray(const Outputs& out, int ray, int group, int which) {
+    if (out.ray_times != nullptr && group == 0) out.ray_times[2 * (size_t)ray + which] = wall_clock64();
+}
+
 // End-of-ray bookkeeping (steam_rt.py:4745-4746, 4848-4849): actual bounces
 // and the active count, per source.
 __device__ __forceinline__ void finish_ray(const RayParams& p, const Outputs& out, const RayState& s, bool alive,
                                            int iter, LaneStats& st) {
+    stamp_ray(out, s.ray, s.group, 1);
     if (s.mask == s.group_mask) {
         if (p.groups > 1 && st.grp != s.group) {
             flush_group_tallies(p, out, st);
@@ -450,6 +458,7 @@ __device__ __forceinline__ bool start_ray(const RayParams& p, const Outputs& out
     for (int b = 0; b < kBands; ++b) s.A[b] = 1.0f;
     s.accum = 0.0f;
     s.mask = s.group_mask;
+    stamp_ray(out, ray, group, 0);
     if (VISUAL && out.vis_limit > 0) {
         const int v = visual_slot(out, ray);
         if (v >= 0) out.vis_rays[v] = ray;
@@ -642,25 +651,27 @@ __device__ __forceinline__ void node_step(const SceneView& sv, unsigned short* s
     }
 }
 
-// The leaf pass of a lane with pending leaves: the triangles of the hit leaf
-// children of node q.pend & 0xFFFF (slots in the bits above), in slot order;
-// a shadow query ends at its first occluder.  The child row is read again
-// (its line was just loaded by the node visit).
+// The leaf pass of a lane with pending leaves: the triangles of the first
+// pending leaf child (lowest slot) of node q.pend & 0xFFFF; the other
+// pending slots stay in q.pend for the next pass, so every pass is one leaf
+// per lane (lanes holding several leaves would otherwise run the pass again
+// with a few lanes).  A shadow query ends at its first occluder.  The leaf's
+// code is read again from the node (its line was just loaded by the visit).
 __device__ __forceinline__ void leaf_pass(const SceneView& sv, Query& q, bool shadow, unsigned& tri_cnt) {
     const int node = q.pend & 0xFFFF;
-    unsigned leaves = (unsigned)q.pend >> 16;
-    q.pend = -1;
-    const global_cfloat* g = (const global_cfloat*)(reinterpret_cast<const char*>(sv.nodes) + ((size_t)node << 7) + 96);
-    const int ch[4] = {__float_as_int(g[0]), __float_as_int(g[1]), __float_as_int(g[2]), __float_as_int(g[3])};
-    while (leaves && !q.occluded) {
-        const int c = __builtin_ctz(leaves);
-        leaves &= leaves - 1u;
-        // branch-free pick of child c
-        const int lo = (c & 1) ? ch[1] : ch[0], hi = (c & 1) ? ch[3] : ch[2];
-        const int code = -(((c & 2) ? hi : lo) + 2);
-        test_leaf(sv, code >> 3, (code & 7) + 1, q, shadow, tri_cnt);
+    const unsigned leaves = (unsigned)q.pend >> 16;
+    const int c = __builtin_ctz(leaves);
+    const unsigned rest = leaves & (leaves - 1u);
+    const global_cfloat* g =
+        (const global_cfloat*)(reinterpret_cast<const char*>(sv.nodes) + ((size_t)node << 7) + 96 + 4 * c);
+    const int code = -(__float_as_int(g[0]) + 2);
+    test_leaf(sv, code >> 3, (code & 7) + 1, q, shadow, tri_cnt);
+    if (q.occluded) {
+        q.pend = -1;
+        q.node = -1;
+    } else {
+        q.pend = rest ? (node | (int)(rest << 16)) : -1;
     }
-    if (q.occluded) q.node = -1;
 }
 
 // A shadow connection from the current hit to one source, with its deposit
@@ -986,7 +997,10 @@ __device__ __forceinline__ bool close_bounce(const SceneView& sv, const RayParam
                 atomicMax(&out.actual_bounces[k], bounce + 1);
             }
         }
-        if (s.mask == 0ull) return false;  // every source has recorded this ray's end
+        if (s.mask == 0ull) {  // every source has recorded this ray's end
+            stamp_ray(out, s.ray, s.group, 1);
+            return false;
+        }
     }
     const float4 nn = sv.normals[h.tri];
     float nx = nn.x, ny = nn.y, nz = nn.z;
@@ -1128,10 +1142,13 @@ __device__ __forceinline__ int claim_rays(bool need, unsigned* cursor, int limit
 // one pass once at least kFinishEighths/8 of its lanes wait for one (of the
 // live lanes once the work pool is drained), or when no lane traverses.  The
 // completion code is long and runs with only the waiting lanes, so it pays to
-// gather most of the wave first; measured on MI355X over 2/8..8/8, 7/8 gave
-// the shortest traces (8/8, pure lockstep, lets the slowest lane hold the
-// wave).
-constexpr int kFinishEighths = 7;
+// gather much of the wave first.  Measured on MI355X over 2/8..8/8 with leaf
+// tests made at once, 7/8 gave the shortest traces (8/8, pure lockstep, lets
+// the slowest lane hold the wave); with gated leaf passes (one leaf per lane
+// and pass), 4/8 / 5/8 / 6/8 / 7/8 / 8/8 gave batched mic arrays 863 / 860 /
+// 838 / 807 / 728 RIRs/s, batched keyframes 1526 / 1543 / 1540 / 1513 /
+// 1354 and a 200k-ray single trace 6.40 / 6.25 / 6.25 / 6.31 / 6.60 ms.
+constexpr int kFinishEighths = 5;
 
 // Steps per loop round of a traversing lane (node visits and leaf passes).
 // Measured on MI355X against one visit per round, with leaves tested at
@@ -1144,13 +1161,16 @@ constexpr int kStepsPerRound = 8;
 
 // Leaf pass gate: the waiting lanes' leaves are tested once
 // waiting * kLeafGateDen >= walking * kLeafGateNum (or no lane walks).
-// Measured on MI355X (waiting : walking = 1:2 / 1:1 / 2:1 / 4:1 / 8:1 /
-// only when none walks): batched mic arrays 746 / 758 / 772 / 768 / 751 /
-// 679 RIRs/s, batched keyframes 1454 / 1457 / 1472 / 1440 / 1414 / 1327,
-// 200k-ray single trace 6.24 / 6.25 / 6.25 / 6.30 / 6.46 / 6.77 ms; leaves
-// tested at once: 681 / 1344 / 6.78 ms.
-constexpr int kLeafGateNum = 2;
-constexpr int kLeafGateDen = 1;
+// Measured on MI355X with the completion threshold at 7/8 and all of a
+// lane's leaves tested in one pass (waiting : walking = 1:2 / 1:1 / 2:1 /
+// 4:1 / 8:1 / only when none walks): batched mic arrays 746 / 758 / 772 /
+// 768 / 751 / 679 RIRs/s, batched keyframes 1454 / 1457 / 1472 / 1440 /
+// 1414 / 1327, 200k-ray single trace 6.24 / 6.25 / 6.25 / 6.30 / 6.46 /
+// 6.77 ms; leaves tested at once: 681 / 1344 / 6.78 ms.  With one leaf per
+// pass and the threshold at 5/8, 3:2 against 2:1: 861 / 859 RIRs/s,
+// 1543 / 1535, 6.27 / 6.28 ms (3:2 kept).
+constexpr int kLeafGateNum = 3;
+constexpr int kLeafGateDen = 2;
 
 // Trace the tasks [first, first + count) handed out by `cursor` (relative to
 // `first`) with the calling wave's lanes.  A lane holds a ray (bounce queries,
@@ -1433,6 +1453,7 @@ inline Outputs make_outputs(const aa_trace_outputs* o, const aa_trace_params* p)
     out.vis_orders = vis ? o->d_visual_orders : nullptr;
     out.vis_dist = vis ? o->d_visual_distances : nullptr;
     out.vis_gain = vis ? o->d_visual_gains : nullptr;
+    out.ray_times = reinterpret_cast<unsigned long long*>(o->d_ray_times);
     return out;
 }
 

@@ -303,11 +303,12 @@ class TraceBuffers:
     log_t: Any
     log_point: Any = None
     visual: tuple | None = None  # points, surfaces, rays, orders, distances, gains (visual candidates)
+    ray_times: Any = None  # [N,2] int64 start / end ticks of the device's 100 MHz clock
 
     @classmethod
     def allocate(cls, device: int, num_sources: int, num_bins: int, num_surfaces: int, ambisonic: bool,
                  log_rays: int = 0, log_bounces: int = 0, visual_limit: int = 0,
-                 num_bounces: int = 0) -> "TraceBuffers":
+                 num_bounces: int = 0, ray_times_rays: int = 0) -> "TraceBuffers":
         torch = _torch()
         dev = f"cuda:{device}"
         f32, i32 = torch.float32, torch.int32
@@ -329,6 +330,7 @@ class TraceBuffers:
             log_t=torch.zeros((log_rays, log_bounces), dtype=f32, device=dev) if log_rays > 0 else None,
             log_point=torch.zeros((log_rays, log_bounces, 3), dtype=f32, device=dev) if log_rays > 0 else None,
             visual=visual,
+            ray_times=torch.zeros((ray_times_rays, 2), dtype=torch.int64, device=dev) if ray_times_rays > 0 else None,
         )
 
     def as_struct(self) -> _lib.TraceOutputs:
@@ -338,7 +340,7 @@ class TraceBuffers:
         vis = [p(t) for t in self.visual] if self.visual is not None else [ctypes.c_void_p(0)] * 6
         return _lib.TraceOutputs(p(self.echogram), p(self.ambisonic), p(self.hit_counts), p(self.contrib_counts),
                                  p(self.surface_energy), p(self.actual_bounces), p(self.active_count), p(self.counters),
-                                 p(self.log_triangle), p(self.log_t), p(self.log_point), *vis)
+                                 p(self.log_triangle), p(self.log_t), p(self.log_point), *vis, p(self.ray_times))
 
 
 def make_params(*, num_rays: int, num_bounces: int, num_bins: int, num_diffuse_samples: int, num_sources: int,
@@ -454,6 +456,7 @@ def trace_energy_field_hip(
     log_bounces: int = 0,
     tuning: Mapping[str, int] | None = None,
     plain: bool = False,
+    ray_times: bool = False,
 ) -> dict[str, Any]:
     if not hip_available(device_id):
         raise RuntimeError(f"HIP device {device_id} is not available")
@@ -507,7 +510,8 @@ def trace_energy_field_hip(
     if V > 0 and (sources is not None and Ks != 1):
         raise ValueError("visual candidates need a single-source trace")
     buffers = TraceBuffers.allocate(device, Ks, int(num_bins), scene.num_surfaces, bool(render_ambisonics),
-                                    int(log_rays), int(log_bounces), visual_limit=V, num_bounces=int(num_bounces))
+                                    int(log_rays), int(log_bounces), visual_limit=V, num_bounces=int(num_bounces),
+                                    ray_times_rays=n_rays if ray_times else 0)
     params = make_params(
         num_rays=n_rays, num_bounces=num_bounces, num_bins=num_bins,
         num_diffuse_samples=int(np.asarray(diffuse_bank).shape[0]), num_sources=Ks,
@@ -550,6 +554,8 @@ def trace_energy_field_hip(
         result["log_triangle"] = buffers.log_triangle.cpu().numpy()
         result["log_t"] = buffers.log_t.cpu().numpy()
         result["log_point"] = buffers.log_point.cpu().numpy()
+    if buffers.ray_times is not None:
+        result["ray_times"] = buffers.ray_times.cpu().numpy()
     result["kernel_time_s"] = kernel_time
     result["transfer_time_s"] = upload_time + (time.perf_counter() - download_started)
     info = device_info(device)

@@ -300,12 +300,23 @@ def c1_single_trace(steps: int, seed: int, plain: bool = False) -> dict:
     torch.cuda.synchronize()
     ms = [a_.elapsed_time(b_) for a_, b_ in ev]
     counters = dict(zip(_lib.COUNTER_NAMES, (int(x) for x in buffers.counters.cpu().numpy())))
+    # one more (untimed) launch with each ray's start and end stamped on the
+    # device's 100 MHz clock: when the rays end, relative to the first start
+    buffers.ray_times = torch.zeros((N, 2), dtype=torch.int64, device="cuda:0")
+    hip_backend.launch_trace(scene, params, src, bank, buffers, stream=stream, plain=plain)
+    torch.cuda.synchronize()
+    t = buffers.ray_times.cpu().numpy().astype(np.float64)
     scene.close()
+    ends = np.sort(t[:, 1] - t[:, 0].min()) * 1e-5  # ticks of 10 ns -> ms
+    ray_ends = {f"p{q:g}": round(float(np.percentile(ends, q)), 3) for q in (50, 90, 99, 99.9)}
+    ray_ends.update(last=round(float(ends[-1]), 3), last_start=round(float((t[:, 0].max() - t[:, 0].min()) * 1e-5), 3),
+                    rays_stamped=int(np.count_nonzero(t[:, 1] >= t[:, 0])))
     return {"workload": f"furnished 3-room L plan, {w.mesh.num_triangles} triangles, 200k rays, 200 bounces, "
                         "2.0 s @ 1 ms" + (", plain one-thread-per-ray kernel" if plain else ""),
             "kernel_ms_p50": round(float(np.percentile(ms, 50)), 4),
             "rirs_per_s": round(1000.0 / float(np.mean(ms)), 2),
-            "primary_rays_per_s": round(N / (float(np.mean(ms)) / 1000.0), 1), "counters": counters}
+            "primary_rays_per_s": round(N / (float(np.mean(ms)) / 1000.0), 1), "counters": counters,
+            "ray_end_ms": ray_ends}
 
 
 def served_report(rates, seconds: float, seed: int, sequential_rates, plain_rates) -> dict:

@@ -168,6 +168,10 @@ typedef struct {
     int32_t *d_visual_orders;     /* [V] bounce + 1 of the strongest contribution (0: none) */
     float *d_visual_distances;    /* [V] its path length */
     float *d_visual_gains;        /* [V] sqrt of its band energy sum */
+    int64_t *d_ray_times;     /* [N,2] when each ray started and ended (source group 0), in ticks of the
+                                 device's constant 100 MHz clock (s_memrealtime), or NULL.  No
+                                 reference counterpart: a diagnostic of how long rays hold their lanes
+                                 (the single-trace tail); a NULL pointer costs nothing. */
 } aa_trace_outputs;
 
 /* d_sources: [Ks,4] device floats (x, y, z, direct_delay_s).

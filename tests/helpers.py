@@ -48,7 +48,7 @@ class Case:
 
 
 def run_gpu(case: Case, *, use_bvh: bool = True, tuning=None, log_rays: int = 0, scene=None, source=None,
-            plain: bool = False, visual_limit: int = 0, visual_stride: int = 1):
+            plain: bool = False, visual_limit: int = 0, visual_stride: int = 1, ray_times: bool = False):
     from acoustic_agent_amd import hip_backend
 
     dirs, bank, U, J = case.ray_set()
@@ -68,7 +68,7 @@ def run_gpu(case: Case, *, use_bvh: bool = True, tuning=None, log_rays: int = 0,
         render_ambisonics=case.ambisonics, device_id=0, rng_mode=case.rng, seed=case.seed,
         num_rays=case.num_rays, scene=scene, tessellation_max_edge=w.max_edge, log_rays=log_rays,
         log_bounces=case.log_bounces if log_rays else 0, tuning=tuning, plain=plain,
-        visual_candidate_limit=visual_limit, visual_stride=visual_stride, **kw)
+        visual_candidate_limit=visual_limit, visual_stride=visual_stride, ray_times=ray_times, **kw)
 
 
 def run_oracle(case: Case, precision: str = "float32", *, triangles: bool = True, log_rays: int = 0,

@@ -251,3 +251,21 @@ def test_motion_frames_in_one_batch_equal_each_frame_alone():
         assert rel_l1(a["ambisonic_echogram"], b["ambisonic_echogram"]) <= 1e-5
         np.testing.assert_allclose(res.rir, alone.rir, rtol=1e-3, atol=1e-6 * float(np.abs(alone.rir).max()))
         assert res.rt60["traced_rt60_bands"] == pytest.approx(alone.rt60["traced_rt60_bands"], rel=1e-3)
+
+
+@pytest.mark.parametrize("plain", [False, True])
+def test_ray_stamps_cover_every_ray_and_change_nothing(shoebox, plain):
+    # per-ray start / end stamps (source group 0) on the device's realtime clock
+    sources = np.asarray([shoebox.source, shoebox.source + np.asarray((0.5, 0.2, 0.0))])
+    case = Case(shoebox, num_rays=5000, num_bounces=30, num_bins=500, seed=7, rng="counter", sources=sources)
+    base = run_gpu(case, plain=plain)
+    stamped = run_gpu(case, plain=plain, ray_times=True)
+    t = stamped["ray_times"]
+    assert t.shape == (5000, 2) and t.dtype == np.int64
+    assert np.all(t[:, 0] > 0) and np.all(t[:, 1] >= t[:, 0])
+    assert "ray_times" not in base
+    for key in ("hit_counts", "contrib_counts", "actual_bounces", "active_count"):
+        assert np.array_equal(base[key], stamped[key]), key
+    for key in ("bounce_queries", "shadow_queries", "contributions"):
+        assert base["counters"][key] == stamped["counters"][key], key
+    assert rel_l1(base["echogram"], stamped["echogram"]) <= 1e-5

@@ -229,3 +229,35 @@ def test_static_batch_rejects_before_touching_the_device():
     assert lib.aa_batch_submit_static(-1, table, 1, ctypes.byref(tuning), None) == _lib.AA_ERR_INVALID
     graph = _lib.BatchTuning(256, 0, 0, 1)
     assert lib.aa_batch_submit_static(0, table, 1, ctypes.byref(graph), None) == _lib.AA_ERR_INVALID
+
+
+def test_ctypes_structures_match_the_c_header(tmp_path):
+    # every field of the ctypes mirrors sits at the offset the C compiler gives
+    # the header's struct (and the sizes agree), so no call passes a shifted field
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    pairs = [("aa_scene_info_t", _lib.SceneInfo), ("aa_trace_params", _lib.TraceParams),
+             ("aa_trace_outputs", _lib.TraceOutputs), ("aa_batch_trace", _lib.BatchTrace),
+             ("aa_batch_tuning", _lib.BatchTuning)]
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "aa_tracer.h"', "int main(void) {"]
+    for cname, cls in pairs:
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in cls._fields_:
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(REPO / "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, cls in pairs:
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for field, _ in cls._fields_:
+            assert got[(cname, field)] == getattr(cls, field).offset, (cname, field)
