@@ -1040,14 +1040,14 @@ __device__ __forceinline__ bool close_bounce(const SceneView& sv, const RayParam
     return true;
 }
 
-// The candidate connections (ray, source) of the wave's waiting rays are
-// spread over its free lanes (lanes not traversing: their
+// Multi-source traces: the candidate connections (ray, source) of the
+// wave's waiting rays are spread over its free lanes (lanes not traversing: their
 // query and connection registers are unused during a completion pass), each
 // free lane computing and parking one connection with the same arithmetic as
 // the ray's own lane would (connect() on the owner's ray and hit, read with
 // lane shuffles).  Without it a ray makes its group's connections one per
 // loop round (about 6 rounds of ~10 lanes per pass on 64-keyframe traces;
-// measured: batched 64-keyframe traces +21 %, single-source 2M rays +2 %).  Takes at most as many
+// measured: batched 64-keyframe traces +21 %).  Takes at most as many
 // as the queue has room for; the rest stay in the owners' c.nee for the
 // per-lane loop.  Every lane of the wave calls it.
 __device__ __forceinline__ void spread_connections(const SceneView& sv, const RayParams& p, const RayState& s,
@@ -1229,30 +1229,30 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
                 // two spreading rounds cover a pass's connections in practice
                 // (measured: one round -6 % on batched 64-keyframe traces,
                 // three no better than two)
-                for (int round = 0; round < 2; ++round)
-                    spread_connections(sv, p, s, c, spread && c.nee != 0ull, !walking, q, cn, qu, q_head, q_count,
-                                       lane, below, st);
+                // single-source rays make their one connection in their own
+                // lane (measured against spreading it: batched mic arrays
+                // +3.3 %, single trace -3.8 % time)
+                if (p.num_sources > 1)
+                    for (int round = 0; round < 2; ++round)
+                        spread_connections(sv, p, s, c, spread && c.nee != 0ull, !walking, q, cn, qu, q_head,
+                                           q_count, lane, below, st);
             }
             // park the new connections while the queue has room; a lane whose
             // connection does not fit resolves it itself before going on
             while (__any(nee)) {
                 bool has = false;
-                if (nee) {
-                    has = next_connection(sv, p, s, c, q, cn, st);
-                    if (!has) {
-                        nee = false;
-                        phase = continue_ray(sv, p, out, s, c, q, st);
-                    }
-                }
+                if (nee) has = next_connection(sv, p, s, c, q, cn, st);
                 // a visual candidate resolves its own connections (its lane
                 // holds the ray state its slot update needs)
                 const bool self = VISUAL && has && out.vis_limit > 0 && visual_slot(out, s.ray) >= 0;
                 const unsigned long long want = __ballot(has && !self);
                 const int room = kQueueCap - q_count;
+                bool parked = false;
                 if (has) {
                     const int rank = __popcll(want & below);
                     if (!self && rank < room) {
                         park(qu, (q_head + q_count + rank) & (kQueueCap - 1), q, cn);
+                        parked = true;
                     } else {
                         phase = kShadow;
                         own = true;
@@ -1260,6 +1260,12 @@ __device__ void trace_rays_refill(const SceneView& sv, const RayParams& p, const
                     }
                 }
                 q_count += min(__popcll(want), room);
+                // the ray goes on once its hit has no connection left to make
+                // (after parking its last one, in the same iteration)
+                if (nee && (!has || (parked && c.nee == 0ull))) {
+                    nee = false;
+                    phase = continue_ray(sv, p, out, s, c, q, st);
+                }
             }
             if (q_count > 0) {
                 wave_sync();  // parked connections are visible to every lane of the wave
