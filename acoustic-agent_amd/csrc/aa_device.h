@@ -23,6 +23,7 @@ namespace aa {
 
 constexpr int kBands = AA_NUM_BANDS;
 constexpr int kStack = 40;          // per-lane traversal stack entries (uint16 node indices in LDS)
+constexpr int kStackSlots = kStack + 1;  // LDS slots per lane: one beyond a full stack (node_step's push)
 constexpr int kMaxWideNodes = 65535; // node indices must fit the 16-bit stack entries and sort keys
 constexpr unsigned kNoChild = 0xFFFFFFFFu;  // sort key of a child that is not pushed
 constexpr float kEps = 1e-6f;       // steam_rt_cuda.py:19
@@ -45,7 +46,7 @@ constexpr int kLogFields = 1 + kBands;
 // stacks (rounded up to 16), the per-wave connection queues, then the
 // per-wave deposit logs (16-byte aligned throughout).
 __host__ __device__ constexpr size_t stack_only_bytes(int wg) {
-    return ((size_t)kStack * wg * sizeof(unsigned short) + 15) & ~(size_t)15;
+    return ((size_t)kStackSlots * wg * sizeof(unsigned short) + 15) & ~(size_t)15;
 }
 __host__ __device__ constexpr size_t log_offset(int wg) {
     return stack_only_bytes(wg) + (size_t)(wg / 64) * kQueueFields * kQueueCap * sizeof(float);
@@ -117,7 +118,7 @@ struct LaneStats {
 
 // LDS views of one workgroup.
 struct Lds {
-    unsigned short* stack;  // kStack x WG
+    unsigned short* stack;  // kStackSlots x WG
     float* queue;           // per wave: kQueueFields x kQueueCap
     float* log;             // per wave: kLogFields x kLogCap (deposits waiting for their atomics)
     float* hist;         // [Ks,6,K] or null
@@ -128,14 +129,22 @@ struct Lds {
 
 __device__ __forceinline__ void philox4x32_10(unsigned c0, unsigned c1, unsigned c2, unsigned c3, unsigned k0,
                                               unsigned k1, unsigned& r0, unsigned& r1) {
+    // The keys are uniform: an empty asm makes them opaque here, so the round
+    // keys are recomputed with scalar adds at each call instead of being
+    // precomputed once and kept in (spilled) scalar registers.
+    asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
     for (int round = 0; round < 10; ++round) {
         if (round > 0) {
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        const unsigned hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-        const unsigned hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        // one 32 x 32 -> 64-bit multiply per product (v_mad_u64_u32), not
+        // separate high and low multiplies
+        const unsigned long long m0 = (unsigned long long)0xD2511F53u * c0;
+        const unsigned long long m1 = (unsigned long long)0xCD9E8D57u * c2;
+        const unsigned hi0 = (unsigned)(m0 >> 32), lo0 = (unsigned)m0;
+        const unsigned hi1 = (unsigned)(m1 >> 32), lo1 = (unsigned)m1;
         const unsigned n0 = hi1 ^ c1 ^ k0;
         const unsigned n2 = hi0 ^ c3 ^ k1;
         c0 = n0;
@@ -632,15 +641,19 @@ __device__ __forceinline__ void node_step(const SceneView& sv, unsigned short* s
     AA_CSWAP(1, 3)
     AA_CSWAP(1, 2)
 #undef AA_CSWAP
-#pragma unroll
-    for (int c = 3; c >= 1; --c) {
-        // the bound is never reached (aa_scene_create caps 3 (depth - 1) at
-        // kStack); it only keeps the LDS accesses in range
-        if (key[c] != kNoChild && q.sp < kStack) {
-            stk[q.sp * WG] = (unsigned short)key[c];
-            ++q.sp;
-        }
-    }
+    // Push key[3], key[2], key[1] (far to near, so the nearest is on top)
+    // without branches: the keys are sorted, so the valid ones are a prefix,
+    // and writing key[3] at sp, key[2] at sp + v3 and key[1] at sp + v3 + v2
+    // (v = valid) puts every valid key above the old top in order, while an
+    // invalid one is overwritten by the next write or left just above the
+    // new top.  The stack never holds more than kStack entries
+    // (aa_scene_create caps 3 (depth - 1) at kStack), and the one slot beyond
+    // it takes the last case's write at a full stack.
+    const int v1 = key[1] != kNoChild, v2 = key[2] != kNoChild, v3 = key[3] != kNoChild;
+    stk[q.sp * WG] = (unsigned short)key[3];
+    stk[(q.sp + v3) * WG] = (unsigned short)key[2];
+    stk[(q.sp + v3 + v2) * WG] = (unsigned short)key[1];
+    q.sp += v1 + v2 + v3;
     if (key[0] != kNoChild) {
         q.node = (int)(key[0] & 0xFFFFu);
     } else if (q.sp > 0) {
