@@ -485,8 +485,12 @@ def run_keyframes(args) -> int:
 
     reqs = workloads.keyframe_batch(num_scenes=args.scenes or 32, seed=args.seed, num_bounces=args.bounces)
     reqs.sort(key=lambda r: -r.work)
-    tuning = Autotuner(0).tune_batch(reqs, sample=4, reps=1, progress=_progress) if not args.no_tune \
-        else BatchTuning()
+    if args.tuning:
+        tuning = BatchTuning(**json.loads(args.tuning))
+    elif args.no_tune:
+        tuning = BatchTuning()
+    else:
+        tuning = Autotuner(0).tune_batch(reqs, sample=4, reps=1, progress=_progress)
     tuning.use_graph = True
     engine = BatchEngine(0, max_traces=len(reqs))
     wall, ms, results = time_batch(engine, reqs, tuning, args.steps, args.warmup)

@@ -231,7 +231,8 @@ typedef struct {
 
 typedef struct {
     int32_t block_size;         /* 64, 128, 256 or 512 lanes per workgroup */
-    int32_t packet_rays;        /* rays per work item */
+    int32_t packet_rays;        /* rays per work item (0: about two work items per resident wave,
+                                   at most 1024 rays) */
     int32_t workgroups_per_cu;  /* persistent grid = CUs x this (0: from LDS use) */
     int32_t use_graph;          /* 1: capture copy + zero + kernel in a hipGraph and replay it */
 } aa_batch_tuning;
