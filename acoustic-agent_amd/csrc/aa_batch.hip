@@ -20,6 +20,8 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <new>
+#include <numeric>
 #include <tuple>
 #include <vector>
 
@@ -50,11 +52,23 @@ struct BatchTrace {
 
 constexpr int kAutoPacketRays = 1024;  // largest packet the automatic rule picks
 
+// Work queues, one per XCD of the MI355X (8 XCDs of 32 CUs, each with its own
+// L2).  Workgroups are dispatched to the XCDs round-robin, so workgroup b
+// runs on XCD b % kQueues; the host gives each queue whole scenes (balanced
+// by packets), and a wave takes packets from its own XCD's queue first, so
+// an XCD's L2 holds the nodes and triangles of one or two scenes at a time
+// instead of every scene in flight on the chip.  A wave whose queue is empty
+// takes packets from the next queues in turn, so the XCDs also finish
+// together.  Items are global packet indices: queue q holds
+// [queue_begin[q], queue_begin[q + 1]).
+constexpr int kQueues = 8;
+
 struct BatchHeader {
     int num_traces;
     int total_packets;
     int packet_rays;
-    unsigned queue_head;
+    int queue_begin[kQueues + 1];
+    unsigned queue_head[kQueues];  // packets claimed from each queue (may run past its end)
 };
 
 __global__ void batch_zero_kernel(const BatchHeader* hdr, const BatchTrace* traces) {
@@ -104,16 +118,28 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
     int* my_contrib = my_hits + surf_cap;
     float* my_energy = reinterpret_cast<float*>(my_contrib + surf_cap);
     const int num_traces = hdr->num_traces;
-    const int total = hdr->total_packets;
     const int packet_rays = hdr->packet_rays;
+    const int xcd = (int)(blockIdx.x % kQueues);
+    int drained = 0;  // wave-uniform: queues (from this XCD's on) found empty
     while (true) {
-        unsigned claimed = 0u;
-        if (lane == 0) claimed = atomicAdd(&hdr->queue_head, 1u);
-        // readfirstlane (not a shuffle) makes the item provably wave-uniform,
-        // so the trace's argument struct below is read with scalar loads into
-        // SGPRs instead of occupying some 60 VGPRs
-        const int item = (int)__builtin_amdgcn_readfirstlane(claimed);
-        if (item >= total) break;  // wave-uniform: every wave exits once the queue is drained
+        int item = -1;
+        while (drained < kQueues) {
+            const int qx = (xcd + drained) % kQueues;
+            unsigned claimed = 0u;
+            if (lane == 0) claimed = atomicAdd(&hdr->queue_head[qx], 1u);
+            // readfirstlane (not a shuffle) makes the item provably
+            // wave-uniform, so the trace's argument struct below is read with
+            // scalar loads into SGPRs instead of occupying some 60 VGPRs
+            const int k = (int)__builtin_amdgcn_readfirstlane(claimed);
+            const int q_begin = __builtin_amdgcn_readfirstlane(hdr->queue_begin[qx]);
+            const int q_end = __builtin_amdgcn_readfirstlane(hdr->queue_begin[qx + 1]);
+            if (k < q_end - q_begin) {
+                item = q_begin + k;
+                break;
+            }
+            ++drained;
+        }
+        if (item < 0) break;  // wave-uniform: every wave exits once every queue is drained
         int lo = 0, hi = num_traces - 1;  // last trace with first_packet <= item
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -167,6 +193,10 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
 
 }  // namespace aa
 
+namespace aa {
+struct FillVectors;
+}
+
 struct aa_batch_s {
     int32_t device = 0;
     int32_t max_traces = 0;
@@ -175,6 +205,7 @@ struct aa_batch_s {
     size_t table_bytes = 0;
     hipEvent_t copied = nullptr;  // the last submission's table copy has finished
     bool pending = false;
+    aa::FillVectors* fill_work = nullptr;  // host work arrays of the table fill (max_traces entries)
     std::map<std::tuple<int, int, int, int, int, int, hipStream_t>, hipGraphExec_t> graphs;
     std::vector<hipGraph_t> graph_objs;
     std::mutex mutex;
@@ -302,16 +333,115 @@ int plan_batch(int device, const aa_batch_trace* traces, int32_t count, const aa
     return AA_OK;
 }
 
+// Host work arrays of fill_table, one entry per trace (owned by the batch
+// object; static for aa_batch_submit_static).
+struct FillWork {
+    int64_t* packets;
+    int32_t* by_scene;       // trace indices sorted by scene, then index
+    int32_t* group_start;    // per scene: first position in by_scene
+    int32_t* group_len;
+    int64_t* group_packets;
+    int32_t* group_order;    // scenes by packets, largest first
+    int32_t* group_queue;
+    int32_t* order;          // table slot -> trace index
+};
+
+template <int N>
+struct FillArrays {
+    int64_t packets[N], group_packets[N];
+    int32_t by_scene[N], group_start[N], group_len[N], group_order[N], group_queue[N], order[N];
+    FillWork view() {
+        return FillWork{packets, by_scene, group_start, group_len, group_packets, group_order, group_queue, order};
+    }
+};
+
+struct FillVectors {
+    std::vector<int64_t> packets, group_packets;
+    std::vector<int32_t> by_scene, group_start, group_len, group_order, group_queue, order;
+    explicit FillVectors(size_t n)
+        : packets(n), group_packets(n), by_scene(n), group_start(n), group_len(n), group_order(n), group_queue(n),
+          order(n) {}
+    FillWork view() {
+        return FillWork{packets.data(),     by_scene.data(),    group_start.data(), group_len.data(),
+                        group_packets.data(), group_order.data(), group_queue.data(), order.data()};
+    }
+};
+
 // The descriptor table (header, then one BatchTrace per trace) of a planned
 // batch, written to host memory `table` of at least table_bytes(count).
-int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& plan, void* table) {
+int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& plan, FillWork& work, void* table) {
     const size_t hdr_bytes = (sizeof(BatchHeader) + 255) & ~size_t(255);
     BatchHeader* h_hdr = static_cast<BatchHeader*>(table);
     BatchTrace* h_tr = reinterpret_cast<BatchTrace*>(static_cast<char*>(table) + hdr_bytes);
-    int64_t packets = 0;
+    // Packets per trace, then the queue of every scene: scenes in order of
+    // their packets (largest first, ties in order of first appearance), each
+    // to the queue with the fewest packets so far; a queue's traces are
+    // contiguous in the table, in the caller's order within a scene.  Only
+    // the caller's work arrays are used (in-place sorts), so the fill
+    // allocates nothing.
+    int64_t* trace_packets = work.packets;
     for (int32_t i = 0; i < count; ++i) {
         const aa_batch_trace& t = traces[i];
-        BatchTrace& d = h_tr[i];
+        const int g = t.params.sources_per_lane > 0 ? std::min(t.params.sources_per_lane, t.params.num_sources)
+                                                    : std::min(8, t.params.num_sources);
+        const int64_t trace_tasks = (int64_t)t.params.num_rays * ((t.params.num_sources + g - 1) / g);
+        trace_packets[i] = (trace_tasks + plan.packet_rays - 1) / plan.packet_rays;
+        work.by_scene[i] = i;
+    }
+    std::sort(work.by_scene, work.by_scene + count, [&](int32_t a, int32_t b) {
+        const auto sa = reinterpret_cast<uintptr_t>(traces[a].scene), sb = reinterpret_cast<uintptr_t>(traces[b].scene);
+        return sa != sb ? sa < sb : a < b;
+    });
+    int32_t groups = 0;
+    for (int32_t k = 0; k < count; ++k) {
+        const int32_t i = work.by_scene[k];
+        if (k == 0 || traces[i].scene != traces[work.by_scene[k - 1]].scene) {
+            work.group_start[groups] = k;
+            work.group_len[groups] = 0;
+            work.group_packets[groups] = 0;
+            work.group_order[groups] = groups;
+            ++groups;
+        }
+        work.group_len[groups - 1] += 1;
+        work.group_packets[groups - 1] += trace_packets[i];
+    }
+    // (a group's first trace is its first appearance: by_scene is sorted by
+    // index within a scene)
+    std::sort(work.group_order, work.group_order + groups, [&](int32_t a, int32_t b) {
+        if (work.group_packets[a] != work.group_packets[b]) return work.group_packets[a] > work.group_packets[b];
+        return work.by_scene[work.group_start[a]] < work.by_scene[work.group_start[b]];
+    });
+    int64_t load[kQueues] = {0};
+    for (int32_t k = 0; k < groups; ++k) {
+        const int32_t g = work.group_order[k];
+        const int q = (int)(std::min_element(load, load + kQueues) - load);
+        work.group_queue[g] = q;
+        load[q] += work.group_packets[g];
+    }
+    int32_t* order = work.order;
+    int32_t placed = 0;
+    int queue_begin[kQueues + 1];
+    {
+        int64_t at = 0;
+        for (int q = 0; q < kQueues; ++q) {
+            queue_begin[q] = (int)at;
+            for (int32_t k = 0; k < groups; ++k) {
+                const int32_t g = work.group_order[k];
+                if (work.group_queue[g] != q) continue;
+                for (int32_t m = 0; m < work.group_len[g]; ++m) {
+                    const int32_t i = work.by_scene[work.group_start[g] + m];
+                    order[placed++] = i;
+                    at += trace_packets[i];
+                }
+            }
+        }
+        queue_begin[kQueues] = (int)at;
+    }
+    int64_t packets = 0;
+    for (int32_t slot = 0; slot < count; ++slot) {
+        const int32_t i = order[slot];
+        const aa_batch_trace& t = traces[i];
+        BatchTrace& d = h_tr[slot];
         std::memset(&d, 0, sizeof(d));
         d.sv = make_scene_view(t.scene, t.params.use_bvh != 0);
         d.p = make_ray_params(&t.params, t.d_sources, t.d_directions, t.d_diffuse_bank, t.d_diffuse_random,
@@ -336,7 +466,8 @@ int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& pla
     h_hdr->num_traces = count;
     h_hdr->total_packets = (int)packets;
     h_hdr->packet_rays = plan.packet_rays;
-    h_hdr->queue_head = 0u;
+    for (int q = 0; q <= kQueues; ++q) h_hdr->queue_begin[q] = queue_begin[q];
+    for (int q = 0; q < kQueues; ++q) h_hdr->queue_head[q] = 0u;
     return AA_OK;
 }
 
@@ -386,6 +517,15 @@ int aa_batch_create(int32_t device, int32_t max_traces, aa_batch_t* out) {
         return fail(AA_ERR_MEMORY, "batch allocation failed: %s", hipGetErrorString(e));
     }
     std::memset(b->h_table, 0, b->table_bytes);
+    try {
+        b->fill_work = new aa::FillVectors((size_t)max_traces);
+    } catch (const std::bad_alloc&) {
+        (void)hipEventDestroy(b->copied);
+        (void)hipHostFree(b->h_table);
+        (void)hipFree(b->d_table);
+        delete b;
+        return fail(AA_ERR_MEMORY, "batch allocation failed (host work arrays)");
+    }
     *out = b;
     return AA_OK;
 }
@@ -399,6 +539,7 @@ int aa_batch_release(aa_batch_t b) {
     (void)hipEventDestroy(b->copied);
     (void)hipHostFree(b->h_table);
     (void)hipFree(b->d_table);
+    delete b->fill_work;
     delete b;
     return AA_OK;
 }
@@ -416,7 +557,8 @@ int aa_batch_submit(aa_batch_t b, const aa_batch_trace* traces, int32_t count, c
 
     // the pinned table may still be read by the previous submission's copy
     if (b->pending) AA_HIP(hipEventSynchronize(b->copied));
-    rc = aa::fill_table(traces, count, plan, b->h_table);
+    aa::FillWork work = b->fill_work->view();
+    rc = aa::fill_table(traces, count, plan, work, b->h_table);
     if (rc != AA_OK) return rc;
 
     auto enqueue = [&](hipStream_t s) -> hipError_t {
@@ -478,6 +620,7 @@ constexpr int kMaxDevices = 64;
 std::mutex g_static_mutex;              // guards the pointers and the host staging table
 void* g_static_table[kMaxDevices] = {};  // device address of `scratch` per device
 alignas(256) unsigned char g_static_staging[kStaticTableBytes];  // host table, .bss (no allocation)
+aa::FillArrays<AA_STATIC_BATCH_TRACES> g_static_fill;  // host work arrays of the table fill, .bss
 
 }  // namespace
 
@@ -501,7 +644,8 @@ extern "C" int aa_batch_submit_static(int32_t device, const aa_batch_trace* trac
                         hipGetErrorString(e));
         g_static_table[device] = ptr;
     }
-    rc = aa::fill_table(traces, count, plan, g_static_staging);
+    aa::FillWork work = g_static_fill.view();
+    rc = aa::fill_table(traces, count, plan, work, g_static_staging);
     if (rc != AA_OK) return rc;
     // from pageable host memory: the staging table has been read when the
     // copy call returns, so the next call may refill it
