@@ -211,7 +211,9 @@ int aa_stream_query(void *stream, int32_t *done);
  *      engine.py:64-73) ----
  * One submission traces `count` independent traces (any mix of scenes, ray
  * counts, bounce limits, bins and source sets) in one persistent launch that
- * pulls packets of `packet_rays` rays from a device work queue.  Each trace's
+ * pulls packets of `packet_rays` rays from device work queues, one per XCD,
+ * each given whole scenes (a wave takes its own XCD's packets first, then
+ * the other queues'), so the order of `traces` does not matter.  Each trace's
  * outputs are zeroed by the submission.  The tuning fields of each trace's
  * aa_trace_params (block_size, lds_*) are ignored; the batch tuning applies.
  * A batch object owns a pinned descriptor table: a new submission on the same
@@ -246,7 +248,7 @@ int aa_batch_release(aa_batch_t batch);
  * For callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid it).  Same traces,
  * tuning and results as aa_batch_submit on `device`, without a batch object:
- * the descriptor table and the work-queue head live in a static device array
+ * the descriptor table and the work-queue heads live in a static device array
  * of AA_STATIC_BATCH_TRACES descriptors (about 100 KB, held by each GPU the
  * library runs on until the process ends), filled from a static host table.
  * Takes 1..AA_STATIC_BATCH_TRACES traces (more: AA_ERR_LIMIT); use_graph must
