@@ -93,7 +93,7 @@ class TraceRequest:
 @dataclass
 class BatchTuning:
     block_size: int = 256
-    packet_rays: int = 0  # 0: about two work items per resident wave, at most 1024 rays
+    packet_rays: int = 0  # 0: guided sizes per trace (large early in each XCD queue, 1024 at its end)
     workgroups_per_cu: int = 0
     use_graph: bool = False
     sources_per_lane: int = 0  # multi-source traces: sources per lane task (0: each trace's own setting)

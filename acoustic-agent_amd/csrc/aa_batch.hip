@@ -45,12 +45,14 @@ struct BatchTrace {
     Outputs out;
     int first_packet;  // prefix sum of packets over the table
     int num_packets;
+    int packet_rays;   // lane tasks per packet of this trace (the last packet may hold fewer)
     int lds_surf;      // single source and S small enough for LDS tallies
     unsigned* zero_ptr[8];
     unsigned zero_words[8];
 };
 
-constexpr int kAutoPacketRays = 1024;  // largest packet the automatic rule picks
+constexpr int kAutoPacketRays = 1024;  // smallest packet of the automatic (guided) sizes on a full GPU
+constexpr int kGuidedMaxRays = 4096;   // largest packet of the guided sizes
 
 // Work queues, one per XCD of the MI355X (8 XCDs of 32 CUs, each with its own
 // L2).  Workgroups are dispatched to the XCDs round-robin, so workgroup b
@@ -118,7 +120,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
     int* my_contrib = my_hits + surf_cap;
     float* my_energy = reinterpret_cast<float*>(my_contrib + surf_cap);
     const int num_traces = hdr->num_traces;
-    const int packet_rays = hdr->packet_rays;
     const int xcd = (int)(blockIdx.x % kQueues);
     int drained = 0;  // wave-uniform: queues (from this XCD's on) found empty
     while (true) {
@@ -153,6 +154,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(3))) void ba
         const RayParams p = uniform_copy(&bt.p);
         const Outputs out = uniform_copy(&bt.out);
         const int first_packet = __builtin_amdgcn_readfirstlane(bt.first_packet);
+        const int packet_rays = __builtin_amdgcn_readfirstlane(bt.packet_rays);
         const bool lds_surf = __builtin_amdgcn_readfirstlane(bt.lds_surf) != 0;
         if (lds_surf) {
             for (int i = lane; i < sv.num_surfaces; i += 64) {
@@ -242,7 +244,27 @@ namespace aa {
 struct BatchPlan {
     int device = 0, wg = 256, waves = 4, surf_cap = 0, per_cu = 1, packet_rays = 0, grid = 1, persistent_grid = 1;
     size_t lds = 0;
+    bool guided = false;          // packet sizes per trace (guided_packet), packet_rays the smallest
+    int64_t waves_per_queue = 1;  // resident waves per XCD work queue
 };
+
+// Guided packet size of a trace whose queue still holds `remaining` lane
+// tasks when the trace's first packet is taken: about 1 / (2 W) of that work
+// for the W waves of the queue, as a power-of-two multiple of the smallest
+// size, at most kGuidedMaxRays.  A wave traces a packet to its last ray before
+// it takes the next one, and the last rays leave most of its lanes idle (on
+// batched mic arrays ~10 % of the waves' time was spent in these drains at
+// 1024-ray packets); large packets early in a queue drain less often, small
+// ones at its end keep the waves finishing together.  Measured on MI355X,
+// batched mic arrays (256 scenes x 4 mics): fixed 1024 / 2048 / 4096 / 8192-ray
+// packets 942 / 922 / 929 / 1008 ms per batch.
+int guided_packet(int64_t remaining, const BatchPlan& plan) {
+    if (!plan.guided) return plan.packet_rays;
+    const int64_t want = remaining / (2 * plan.waves_per_queue);
+    int64_t size = plan.packet_rays;
+    while (size * 2 <= kGuidedMaxRays && size * 2 <= want) size *= 2;
+    return (int)size;
+}
 
 size_t table_bytes(int traces) {
     const size_t hdr = (sizeof(BatchHeader) + 255) & ~size_t(255);
@@ -299,12 +321,13 @@ int plan_batch(int device, const aa_batch_trace* traces, int32_t count, const aa
         }
     }
     plan.per_cu = per_cu;
-    // packet size: the tuning's, or (0) about two work items per resident
-    // wave so that small batches still spread over the whole GPU, at most
-    // kAutoPacketRays so that large batches keep enough work items to balance
-    // uneven traces (measured on 256 scenes x 4-mic arrays, 1024 traces:
-    // 1024-ray packets 1122 ms per batch, the 16384 of two items per wave
-    // 1300 ms)
+    // packet size: the tuning's, or (0) guided sizes per trace (guided_packet)
+    // whose smallest is about two work items per resident wave, so that small
+    // batches still spread over the whole GPU, at most kAutoPacketRays, so
+    // that the end of a large batch keeps enough work items to balance uneven
+    // traces (measured on 256 scenes x 4-mic arrays, 1024 traces, with one
+    // size for the whole batch: 1024-ray packets 1122 ms per batch, the 16384
+    // of two items per wave 1300 ms)
     int64_t tasks = 0;
     for (int32_t i = 0; i < count; ++i) {
         const aa_batch_trace& t = traces[i];
@@ -313,12 +336,18 @@ int plan_batch(int device, const aa_batch_trace* traces, int32_t count, const aa
         tasks += (int64_t)t.params.num_rays * ((t.params.num_sources + g - 1) / g);
     }
     int packet_rays = tuning->packet_rays;
+    const int64_t resident = (int64_t)props.cu * per_cu * plan.waves;
+    plan.waves_per_queue = std::max<int64_t>(1, resident / kQueues);
     if (packet_rays <= 0) {
-        const int64_t resident = (int64_t)props.cu * per_cu * plan.waves;
+        // the smallest guided size: about two work items per resident wave
+        // (small batches still spread over the whole GPU), at most
+        // kAutoPacketRays
         const int64_t want = (tasks + 2 * resident - 1) / (2 * resident);
         packet_rays = (int)std::min<int64_t>(kAutoPacketRays, std::max<int64_t>(64, (want + 63) / 64 * 64));
+        plan.guided = true;
     }
     plan.packet_rays = packet_rays;
+    // packets at the smallest size: an upper bound under guided sizes
     int64_t packets = 0;
     for (int32_t i = 0; i < count; ++i) {
         const aa_batch_trace& t = traces[i];
@@ -336,11 +365,11 @@ int plan_batch(int device, const aa_batch_trace* traces, int32_t count, const aa
 // Host work arrays of fill_table, one entry per trace (owned by the batch
 // object; static for aa_batch_submit_static).
 struct FillWork {
-    int64_t* packets;
+    int64_t* tasks;          // lane tasks per trace
     int32_t* by_scene;       // trace indices sorted by scene, then index
     int32_t* group_start;    // per scene: first position in by_scene
     int32_t* group_len;
-    int64_t* group_packets;
+    int64_t* group_tasks;
     int32_t* group_order;    // scenes by packets, largest first
     int32_t* group_queue;
     int32_t* order;          // table slot -> trace index
@@ -348,22 +377,22 @@ struct FillWork {
 
 template <int N>
 struct FillArrays {
-    int64_t packets[N], group_packets[N];
+    int64_t tasks[N], group_tasks[N];
     int32_t by_scene[N], group_start[N], group_len[N], group_order[N], group_queue[N], order[N];
     FillWork view() {
-        return FillWork{packets, by_scene, group_start, group_len, group_packets, group_order, group_queue, order};
+        return FillWork{tasks, by_scene, group_start, group_len, group_tasks, group_order, group_queue, order};
     }
 };
 
 struct FillVectors {
-    std::vector<int64_t> packets, group_packets;
+    std::vector<int64_t> tasks, group_tasks;
     std::vector<int32_t> by_scene, group_start, group_len, group_order, group_queue, order;
     explicit FillVectors(size_t n)
-        : packets(n), group_packets(n), by_scene(n), group_start(n), group_len(n), group_order(n), group_queue(n),
+        : tasks(n), group_tasks(n), by_scene(n), group_start(n), group_len(n), group_order(n), group_queue(n),
           order(n) {}
     FillWork view() {
-        return FillWork{packets.data(),     by_scene.data(),    group_start.data(), group_len.data(),
-                        group_packets.data(), group_order.data(), group_queue.data(), order.data()};
+        return FillWork{tasks.data(),     by_scene.data(),    group_start.data(), group_len.data(),
+                        group_tasks.data(), group_order.data(), group_queue.data(), order.data()};
     }
 };
 
@@ -373,19 +402,19 @@ int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& pla
     const size_t hdr_bytes = (sizeof(BatchHeader) + 255) & ~size_t(255);
     BatchHeader* h_hdr = static_cast<BatchHeader*>(table);
     BatchTrace* h_tr = reinterpret_cast<BatchTrace*>(static_cast<char*>(table) + hdr_bytes);
-    // Packets per trace, then the queue of every scene: scenes in order of
-    // their packets (largest first, ties in order of first appearance), each
-    // to the queue with the fewest packets so far; a queue's traces are
-    // contiguous in the table, in the caller's order within a scene.  Only
-    // the caller's work arrays are used (in-place sorts), so the fill
-    // allocates nothing.
-    int64_t* trace_packets = work.packets;
+    // Lane tasks per trace, then the queue of every scene: scenes in order of
+    // their tasks (largest first, ties in order of first appearance), each
+    // to the queue with the fewest tasks so far; a queue's traces are
+    // contiguous in the table, in the caller's order within a scene, and
+    // each trace's packet size follows the work its queue still holds at the
+    // trace (guided_packet).  Only the caller's work arrays are used
+    // (in-place sorts), so the fill allocates nothing.
+    int64_t* task_count = work.tasks;
     for (int32_t i = 0; i < count; ++i) {
         const aa_batch_trace& t = traces[i];
         const int g = t.params.sources_per_lane > 0 ? std::min(t.params.sources_per_lane, t.params.num_sources)
                                                     : std::min(8, t.params.num_sources);
-        const int64_t trace_tasks = (int64_t)t.params.num_rays * ((t.params.num_sources + g - 1) / g);
-        trace_packets[i] = (trace_tasks + plan.packet_rays - 1) / plan.packet_rays;
+        task_count[i] = (int64_t)t.params.num_rays * ((t.params.num_sources + g - 1) / g);
         work.by_scene[i] = i;
     }
     std::sort(work.by_scene, work.by_scene + count, [&](int32_t a, int32_t b) {
@@ -398,17 +427,17 @@ int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& pla
         if (k == 0 || traces[i].scene != traces[work.by_scene[k - 1]].scene) {
             work.group_start[groups] = k;
             work.group_len[groups] = 0;
-            work.group_packets[groups] = 0;
+            work.group_tasks[groups] = 0;
             work.group_order[groups] = groups;
             ++groups;
         }
         work.group_len[groups - 1] += 1;
-        work.group_packets[groups - 1] += trace_packets[i];
+        work.group_tasks[groups - 1] += task_count[i];
     }
     // (a group's first trace is its first appearance: by_scene is sorted by
     // index within a scene)
     std::sort(work.group_order, work.group_order + groups, [&](int32_t a, int32_t b) {
-        if (work.group_packets[a] != work.group_packets[b]) return work.group_packets[a] > work.group_packets[b];
+        if (work.group_tasks[a] != work.group_tasks[b]) return work.group_tasks[a] > work.group_tasks[b];
         return work.by_scene[work.group_start[a]] < work.by_scene[work.group_start[b]];
     });
     int64_t load[kQueues] = {0};
@@ -416,29 +445,31 @@ int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& pla
         const int32_t g = work.group_order[k];
         const int q = (int)(std::min_element(load, load + kQueues) - load);
         work.group_queue[g] = q;
-        load[q] += work.group_packets[g];
+        load[q] += work.group_tasks[g];
     }
     int32_t* order = work.order;
     int32_t placed = 0;
-    int queue_begin[kQueues + 1];
-    {
-        int64_t at = 0;
-        for (int q = 0; q < kQueues; ++q) {
-            queue_begin[q] = (int)at;
-            for (int32_t k = 0; k < groups; ++k) {
-                const int32_t g = work.group_order[k];
-                if (work.group_queue[g] != q) continue;
-                for (int32_t m = 0; m < work.group_len[g]; ++m) {
-                    const int32_t i = work.by_scene[work.group_start[g] + m];
-                    order[placed++] = i;
-                    at += trace_packets[i];
-                }
-            }
+    int32_t queue_slot[kQueues + 1];  // first table slot of each queue
+    for (int q = 0; q < kQueues; ++q) {
+        queue_slot[q] = placed;
+        for (int32_t k = 0; k < groups; ++k) {
+            const int32_t g = work.group_order[k];
+            if (work.group_queue[g] != q) continue;
+            for (int32_t m = 0; m < work.group_len[g]; ++m) order[placed++] = work.by_scene[work.group_start[g] + m];
         }
-        queue_begin[kQueues] = (int)at;
     }
+    queue_slot[kQueues] = placed;
+    int queue_begin[kQueues + 1];
     int64_t packets = 0;
+    int q = 0;
+    int64_t queued = 0;  // tasks of queue q in the slots before this one
+    queue_begin[0] = 0;
     for (int32_t slot = 0; slot < count; ++slot) {
+        while (slot >= queue_slot[q + 1]) {  // the next non-empty queue starts here
+            ++q;
+            queue_begin[q] = (int)packets;
+            queued = 0;
+        }
         const int32_t i = order[slot];
         const aa_batch_trace& t = traces[i];
         BatchTrace& d = h_tr[slot];
@@ -453,9 +484,11 @@ int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& pla
         d.out = make_outputs(&t.outputs, &t.params);
         const int64_t trace_tasks = (int64_t)d.p.num_rays * d.p.groups;
         if (trace_tasks > (int64_t)1 << 30) return fail(AA_ERR_LIMIT, "trace %d has too many lane tasks", i);
-        d.num_packets = (int)((trace_tasks + plan.packet_rays - 1) / plan.packet_rays);
+        d.packet_rays = guided_packet(load[q] - queued, plan);
+        d.num_packets = (int)((trace_tasks + d.packet_rays - 1) / d.packet_rays);
         d.first_packet = (int)packets;
         packets += d.num_packets;
+        queued += trace_tasks;
         d.lds_surf = (t.params.num_sources == 1 && t.scene->num_surfaces <= plan.surf_cap) ? 1 : 0;
         const OutputExtents ext = output_extents(&t.outputs, &t.params, t.scene->num_surfaces);
         for (int r = 0; r < 8; ++r) {
@@ -463,6 +496,7 @@ int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& pla
             d.zero_words[r] = (unsigned)(ext.bytes[r] / 4);
         }
     }
+    while (q < kQueues) queue_begin[++q] = (int)packets;
     h_hdr->num_traces = count;
     h_hdr->total_packets = (int)packets;
     h_hdr->packet_rays = plan.packet_rays;

@@ -62,6 +62,11 @@ def test_batch_space_covers_the_tuning_fields():
     assert {"block_size", "packet_rays", "workgroups_per_cu", "sources_per_lane"} <= set(BATCH_SPACE)
     assert set(MULTI_SOURCE_AXES) <= set(BATCH_SPACE)
     for k, values in BATCH_SPACE.items():
+        if k == "packet_rays":
+            # held at the guided sizes (0): a sample cannot show the queue
+            # drains they remove from the full batch
+            assert tuple(values) == (0,), k
+            continue
         assert len(values) == len(set(values)) >= 2, k
     assert BATCH_SPACE["block_size"][0] == BatchTuning().block_size
     assert BATCH_SPACE["packet_rays"][0] == BatchTuning().packet_rays
