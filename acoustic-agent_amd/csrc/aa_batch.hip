@@ -244,25 +244,31 @@ namespace aa {
 struct BatchPlan {
     int device = 0, wg = 256, waves = 4, surf_cap = 0, per_cu = 1, packet_rays = 0, grid = 1, persistent_grid = 1;
     size_t lds = 0;
-    bool guided = false;          // packet sizes per trace (guided_packet), packet_rays the smallest
-    int64_t waves_per_queue = 1;  // resident waves per XCD work queue
+    bool guided = false;     // packet sizes per trace (guided_packet), packet_rays the smallest
+    int64_t resident = 1;    // resident waves of the launch
 };
 
-// Guided packet size of a trace whose queue still holds `remaining` lane
-// tasks when the trace's first packet is taken: about 1 / (2 W) of that work
-// for the W waves of the queue, as a power-of-two multiple of the smallest
-// size, at most kGuidedMaxRays.  A wave traces a packet to its last ray before
-// it takes the next one, and the last rays leave most of its lanes idle (on
-// batched mic arrays ~10 % of the waves' time was spent in these drains at
-// 1024-ray packets); large packets early in a queue drain less often, small
-// ones at its end keep the waves finishing together.  Measured on MI355X,
-// batched mic arrays (256 scenes x 4 mics): fixed 1024 / 2048 / 4096 / 8192-ray
-// packets 942 / 922 / 929 / 1008 ms per batch.
-int guided_packet(int64_t remaining, const BatchPlan& plan) {
-    if (!plan.guided) return plan.packet_rays;
-    const int64_t want = remaining / (2 * plan.waves_per_queue);
+// Guided packet size of a trace whose queue (of `queue_tasks` lane tasks)
+// still holds `remaining` when the trace's first packet is taken, in a batch
+// of `batch_tasks`: the queues advance together (waves steal from the other
+// queues once their own is empty), so about remaining / queue_tasks of the
+// batch is left, and a packet takes 1 / (2 W) of that for the W resident
+// waves; as a power-of-two multiple of the smallest size, at most
+// kGuidedMaxRays.  A wave traces a packet to its last ray before it takes
+// the next one, and the last rays leave most of its lanes idle (on batched
+// mic arrays ~10 % of the waves' time was spent in these drains at 1024-ray
+// packets); large packets early in a batch drain less often, small ones at
+// its end keep the waves finishing together, and a small or lopsided batch
+// (a served batch of a few scenes, which fill one or two queues) keeps the
+// smallest size.  Measured on MI355X, batched mic arrays (256 scenes x 4
+// mics): fixed 1024 / 2048 / 4096 / 8192-ray packets 942 / 922 / 929 / 1008 ms
+// per batch, guided 887 ms.
+int guided_packet(int64_t remaining, int64_t queue_tasks, int64_t batch_tasks, const BatchPlan& plan) {
+    if (!plan.guided || queue_tasks <= 0) return plan.packet_rays;
+    const double left = (double)remaining / (double)queue_tasks * (double)batch_tasks;
+    const double want = left / (2.0 * (double)plan.resident);
     int64_t size = plan.packet_rays;
-    while (size * 2 <= kGuidedMaxRays && size * 2 <= want) size *= 2;
+    while (size * 2 <= kGuidedMaxRays && (double)(size * 2) <= want) size *= 2;
     return (int)size;
 }
 
@@ -337,7 +343,7 @@ int plan_batch(int device, const aa_batch_trace* traces, int32_t count, const aa
     }
     int packet_rays = tuning->packet_rays;
     const int64_t resident = (int64_t)props.cu * per_cu * plan.waves;
-    plan.waves_per_queue = std::max<int64_t>(1, resident / kQueues);
+    plan.resident = std::max<int64_t>(1, resident);
     if (packet_rays <= 0) {
         // the smallest guided size: about two work items per resident wave
         // (small batches still spread over the whole GPU), at most
@@ -441,6 +447,8 @@ int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& pla
         return work.by_scene[work.group_start[a]] < work.by_scene[work.group_start[b]];
     });
     int64_t load[kQueues] = {0};
+    int64_t batch_tasks = 0;
+    for (int32_t i = 0; i < count; ++i) batch_tasks += task_count[i];
     for (int32_t k = 0; k < groups; ++k) {
         const int32_t g = work.group_order[k];
         const int q = (int)(std::min_element(load, load + kQueues) - load);
@@ -484,7 +492,7 @@ int fill_table(const aa_batch_trace* traces, int32_t count, const BatchPlan& pla
         d.out = make_outputs(&t.outputs, &t.params);
         const int64_t trace_tasks = (int64_t)d.p.num_rays * d.p.groups;
         if (trace_tasks > (int64_t)1 << 30) return fail(AA_ERR_LIMIT, "trace %d has too many lane tasks", i);
-        d.packet_rays = guided_packet(load[q] - queued, plan);
+        d.packet_rays = guided_packet(load[q] - queued, load[q], batch_tasks, plan);
         d.num_packets = (int)((trace_tasks + d.packet_rays - 1) / d.packet_rays);
         d.first_packet = (int)packets;
         packets += d.num_packets;
