@@ -74,6 +74,32 @@ def test_batch_tunings_agree(requests, tuning):
     engine.close()
 
 
+def test_guided_packets_equal_fixed_packets():
+    """A batch large enough for guided packets of several sizes: 64 traces of
+    400k rays over two scenes (2 of the 8 XCD queues), 25.6 M lane tasks, so
+    on 3072 resident waves the first traces of each queue get 4096-ray packets
+    (25.6 M / (2 x 3072) = 4167 > 4096) and the last ones 1024.  Every trace's
+    result equals the fixed 1024-ray packets' (order of float sums aside)."""
+    box = scenes.shoebox_workload(0)
+    plan = scenes.l_plan_workload(1, target_triangles=2000)
+    s_box, s_plan = _scene(box), _scene(plan)
+    rng = np.random.default_rng(11)
+    reqs = []
+    for i in range(64):
+        w, s = (box, s_box) if i % 2 else (plan, s_plan)
+        reqs.append(TraceRequest(scene=s, listener=w.listener + rng.uniform(-0.05, 0.05, 3), sources=w.source[None, :],
+                                 num_rays=400_000, num_bounces=3, duration_s=0.5,
+                                 seed=int(rng.integers(0, 1 << 40)), tag=i))
+    engine = BatchEngine(0, max_traces=64)
+    guided = {r["tag"]: r for r in engine.submit(reqs, BatchTuning(packet_rays=0)).wait()}
+    fixed = {r["tag"]: r for r in engine.submit(reqs, BatchTuning(packet_rays=1024)).wait()}
+    engine.close()
+    assert sorted(guided) == list(range(64))
+    for tag, r in fixed.items():
+        _check_same(guided[tag], r)
+        assert r["counters"]["bounce_queries"] > 400_000
+
+
 def test_graph_replays_follow_new_contents(requests):
     import torch
 
