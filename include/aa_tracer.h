@@ -211,8 +211,8 @@ int aa_stream_query(void *stream, int32_t *done);
  *      engine.py:64-73) ----
  * One submission traces `count` independent traces (any mix of scenes, ray
  * counts, bounce limits, bins and source sets) in one persistent launch that
- * pulls packets of rays (`packet_rays`, or guided sizes) from device work queues, one per XCD,
- * each given whole scenes (a wave takes its own XCD's packets first, then
+ * pulls packets of rays (`packet_rays`, or guided sizes) from device work
+ * queues, one per XCD, each given whole scenes (a wave takes its own XCD's packets first, then
  * the other queues'), so the order of `traces` does not matter.  Each trace's
  * outputs are zeroed by the submission.  The tuning fields of each trace's
  * aa_trace_params (block_size, lds_*) are ignored; the batch tuning applies.
@@ -234,9 +234,10 @@ typedef struct {
 typedef struct {
     int32_t block_size;         /* 64, 128, 256 or 512 lanes per workgroup */
     int32_t packet_rays;        /* lane tasks (rays x source groups) per work item; 0: guided
-                                   sizes per trace, from about 1 / (2 W) of the work its XCD
-                                   queue still holds for the queue's W resident waves, between
-                                   min(1024, two work items per resident wave) and 4096 */
+                                   sizes per trace, about 1 / (2 W) of the batch's work still
+                                   left at the trace's place in its XCD queue, for the launch's
+                                   W resident waves, as a power-of-two multiple of the smallest,
+                                   min(1024, two work items per resident wave), at most 4096 */
     int32_t workgroups_per_cu;  /* persistent grid = CUs x this (0: from LDS use) */
     int32_t use_graph;          /* 1: capture copy + zero + kernel in a hipGraph and replay it */
 } aa_batch_tuning;
